@@ -1,0 +1,159 @@
+#!/usr/bin/env python3
+"""DEM-MPH explicit step on one MI355X: particle-steps/s on the large single scene.
+
+Workload (``dam_break_c2``): 3D dam break in a 0.8 x 0.4 x 0.6 m tank at
+d = 0.004 m -- 562,500 fluid particles, 340,908 wall particles (three layers,
+open top) and 64 rigid debris bodies of 10x10x10 particles with seeded
+densities in [500, 900] kg/m^3 -- 967,408 particles, dt = 2e-5 s.  FP32 pair
+arithmetic on cell-relative positions, FP64 rigid-body reductions/integration
+and FP64 time.  Inputs are resident in HBM before the timed region.
+
+One JSON line on stdout:
+  value      particle-steps per second over the timed steps (whole job = one GPU)
+  roofline   the dominant kernel's algorithmic HBM bytes per launch over its
+             average duration (HIP events on the library's stream), vs 8 TB/s
+  cpu_baseline  the FP64 oracle (this package's C restatement of the reference,
+             OpenMP on the host cores) on a bounded sample of the same scene
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "particle-steps/s and sim-s/wall-s on 1 MI355X, plain vs mixed/batched/learned"
+
+# Algorithmic HBM bytes per particle per launch (FP32 state): each slot's own
+# inputs read once and outputs written once; neighbour gathers re-read arrays
+# already counted and are served from cache in the ideal.
+#   sweep1: pos 12 + vel 12 + type 4 + key 4 + rho 4 read; VolStrainP, DivergenceP,
+#           PressureA, mass, kappa, PressureP written (6 x 4)            = 36 + 24 = 60
+#   sweep2 (fluid + solid slots): pos 12 + vel 12 + type 4 + key 4 + PressureP 4
+#           + rho 4 + mass 4 read; force 12 written                      = 44 + 12 = 56
+#   dem (solid + wall slots): pos 12 + vel 12 + type 4 + key 4 + body 4 + force 12
+#           + four coefficients 16 read; force 12 + coefficients 16 written = 64 + 28 = 92
+ALGO_BYTES = {"sweep1": 60.0, "sweep2": 56.0, "dem": 92.0}
+
+
+def build_workload(seed: int = 1):
+    from dem_mph_amd import scenes as S
+    sc = S.dam_break_c2(seed=seed, rigid=True)
+    return sc, S.params_c2()
+
+
+def cpu_baseline(sc, params, budget_s: float = 20.0):
+    """The FP64 oracle (OpenMP, all host threads) on the same scene: set-up is
+    not timed; steps are run until about `budget_s` seconds of wall time."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as P
+    if not os.path.exists(P.LIB_PATH):
+        import subprocess
+        subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "oracle"], check=True, capture_output=True)
+    threads = os.cpu_count() or 1
+    try:
+        threads = len(os.sched_getaffinity(0))
+    except Exception:
+        pass
+    threads = int(os.environ.get("OMP_NUM_THREADS", threads))
+    o = P.Oracle(P.params_from_dict(params), sc, sort_mode=P.SORT_CANONICAL, rigid=True, threads=threads)
+    steps = 0
+    t0 = time.perf_counter()
+    while True:
+        o.step(1)
+        steps += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s or steps >= 50:
+            break
+    o.close()
+    return {"value": sc.n * steps / el, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+            "sample": "FP64 oracle, rigid debris, same %d-particle scene, %d steps after set-up (%.1f s)"
+                      % (sc.n, steps, el)}
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--precision", choices=("fp32", "fp64"), default="fp32")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--seed", type=int, default=1)
+    a = ap.parse_args(argv)
+    if a.gpus != 1:
+        print("bench.py measures one GPU: the DEM-MPH step here runs on a single MI355X "
+              "(multi-GPU slab decomposition is designed, not built); use --gpus 1", file=sys.stderr)
+        return 2
+
+    import dem_mph_amd as D
+    sc, params = build_workload(a.seed)
+    prec = D.PREC_FP32 if a.precision == "fp32" else D.PREC_FP64
+    print("bench: building solver for %d particles (%s)" % (sc.n, a.precision), file=sys.stderr, flush=True)
+    s = D.Solver(D.Params.from_dict(params), sc, precision=prec, rigid=True)
+    info = s.info()
+    s.step(a.warmup)
+    s.sync()
+    s.set_timing(True)
+    t0 = time.perf_counter()
+    s.step(a.steps)
+    s.sync()
+    el = time.perf_counter() - t0
+    tm = s.timing()
+    reb0 = info.rebuilds
+    reb = s.info().rebuilds
+    s.set_timing(False)
+    n = sc.n
+    pss = n * a.steps / el
+    dt = params["Dt"]
+    # dominant kernel: the one with the largest total device time
+    groups = {k: v for k, v in tm.items() if v[1] > 0}
+    dom = max(("sweep1", "sweep2", "dem"), key=lambda k: groups.get(k, (0, 0))[0])
+    ms, launches = groups[dom]
+    units = {"sweep1": n, "sweep2": info.solid_end - info.fluid_begin, "dem": info.wall_end - info.solid_begin}[dom]
+    avg_s = ms / 1e3 / launches
+    achieved = ALGO_BYTES[dom] * units / avg_s / 1e9
+    out = {
+        "metric": METRIC,
+        "value": pss,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": el / a.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if prec == D.PREC_FP32 else "f64",
+        "data": "synthetic (seeded dam-break scene, jitter +-0.01 d)",
+        "config": {"workload": "dam_break_c2: 3D tank 0.8x0.4x0.6 m, d=0.004 m, %d particles "
+                               "(562500 fluid, 64 rigid debris bodies of 10^3, 340908 wall), dt=2e-5 s" % n,
+                   "particles": n, "bodies": info.nbodies, "precision": a.precision,
+                   "rigid_body_math": "f64", "sort": "canonical", "rebuild": "skin 0.05 d"},
+        "sim_s_per_wall_s": dt * a.steps / el,
+        "rebuilds_in_timed_region": int(reb - reb0),
+        "kernel_ms_per_step": {k: v[0] / a.steps for k, v in groups.items()},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "algorithmic_bytes_per_unit": ALGO_BYTES[dom], "units_per_launch": units,
+                     "avg_launch_ms": avg_s * 1e3},
+    }
+    s.close()
+    if not a.no_cpu_baseline:
+        print("bench: timing the CPU baseline (bounded sample)", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(sc, params, a.cpu_budget)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,97 @@
+// Device-side data layout and launcher declarations for the DEM-MPH step.
+//
+// Particles live in "slots": structure-of-arrays sorted by the reference's cell
+// key (cell id + block * cells, block 0 fluid / 1 solid / 2 wall;
+// source/DEM-MPH.cpp:1292-1313).  Block ranges never change after the first
+// binning (the type counts are fixed), so [fb,fe) [sb,se) [wb,we) are host
+// constants.  Per-particle state is permuted at every rebinning; the derived
+// per-slot arrays (pressures, DEM coefficients, ...) stay where they are, as
+// the reference's do.
+//
+// Positions: with Real = double they are absolute, as in the reference; with
+// Real = float they are offsets from the origin of the particle's cell at the
+// last binning, so a pair separation is shift*CellWidth + (o_j - o_i) with the
+// stencil shift in {-1, 0, 1}: no periodic Mod, and FP32 resolves ~1e-9 m over
+// a cell instead of ~1e-7 m over the domain.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace dmph {
+
+constexpr int kNType = 6;
+
+// Uniform parameters of one scene (passed by value to every kernel).
+struct Geo {
+    int n, ncell, dim, nb, substeps;
+    int cc[3];
+    int fb, fe, sb, se, wb, we;
+    int st_active;      // any non-zero InteractionRatio: surface-tension passes contribute
+    int rigid;          // solid particles with body >= 0 move as rigid bodies
+    int sort_reference; // within-cell order from the reference's bitonic network
+    int rebuild_every;  // rebin every step instead of the skin test
+    int p2, p2_log;     // bitonic size (power of two > n) and its log2
+    int compensated;    // Kahan sums (FP32)
+    double cw[3], dmin[3], dw[3];
+    double d, vol, dt, dem_dt;
+    double ra, rg, rp, rv, rmax, cut2, skin2;
+    double swa, swg, swp, swv, n0a, n0p, r2g, cofk;
+    double cofa[kNType], dens[kNType], bulk[kNType], bvisc[kNType], shear[kNType];
+    double young[kNType], poisson[kNType], logrest[kNType], fric[kNType];
+    double inter[kNType * kNType];
+    double grav[3];
+    double wall_rot[2][9], wall_omega[2][3], wall_vel[2][3];  // wall types 4 and 5
+};
+
+// Device scalars shared by all kernels of a step.
+struct Scalars {
+    double time;
+    unsigned long long skin_bits;   // max squared displacement since the last binning (as bits)
+    int flag;                       // rebin this step
+    int force;                      // rebin unconditionally at the next decision (first binning)
+    long long rebuilds;
+    double wall_center[2][3];
+};
+
+template <class R>
+struct Dev {
+    // per-particle state (permuted on rebinning)
+    int *id, *type, *body, *key;
+    R *px, *py, *pz, *vx, *vy, *vz, *rho;
+    double *r0;                    // [3n] body-frame offsets (rigid)
+    // per-slot values (not permuted)
+    R *fx, *fy, *fz;
+    R *bx, *by, *bz;               // positions at the last binning
+    R *dens_a, *gcx, *gcy, *gcz, *vs_p, *div_p, *press_p, *press_a, *mass, *kappa, *alpha, *drag_c;
+    R *kn, *etan, *kt, *etat, *virial_p;
+    int *ncount, *scount;
+    int *cell_start;               // [3*ncell + 1]
+    // rebinning scratch
+    int *counts, *fill;            // [3*ncell + 1]
+    int *key_new, *order, *perm;   // [n]
+    int *bkey, *bval;              // [p2] bitonic
+    int *block_sums;               // scan carries
+    int *t_int;                    // [4n] gather scratch
+    R *t_real;                     // [8n]
+    double *t_dbl;                 // [3n]
+    // rigid bodies
+    double *b_mass, *b_x, *b_v, *b_l, *b_q, *b_iinv, *b_f, *b_t, *b_w, *b_r;
+    Scalars *sc;
+};
+
+// Launchers (defined in dmph_kernels.inc, instantiated for float and double).
+template <class R> void launch_predict(const Geo &g, const Dev<R> &a, hipStream_t s);
+template <class R> void launch_rebin(const Geo &g, const Dev<R> &a, hipStream_t s);
+template <class R> void launch_sweep1(const Geo &g, const Dev<R> &a, hipStream_t s);
+template <class R> void launch_sweep2(const Geo &g, const Dev<R> &a, hipStream_t s);
+template <class R> void launch_accel(const Geo &g, const Dev<R> &a, hipStream_t s);
+template <class R> void launch_dem(const Geo &g, const Dev<R> &a, hipStream_t s);
+template <class R> void launch_move(const Geo &g, const Dev<R> &a, hipStream_t s);
+template <class R> void launch_step_end(const Geo &g, const Dev<R> &a, hipStream_t s);
+template <class R> void launch_virial(const Geo &g, const Dev<R> &a, hipStream_t s);
+template <class R> void launch_pairs(const Geo &g, const Dev<R> &a, int64_t *row_counts, const int64_t *ptr, int *idx,
+                                     hipStream_t s);
+template <class R> void launch_init_density(const Geo &g, const Dev<R> &a, hipStream_t s);
+
+}  // namespace dmph
