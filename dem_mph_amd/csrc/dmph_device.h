@@ -33,6 +33,7 @@ struct Geo {
     int rebuild_every;  // rebin every step instead of the skin test
     int p2, p2_log;     // bitonic size (power of two > n) and its log2
     int compensated;    // Kahan sums (FP32)
+    int kcap;           // neighbour-list capacity per particle
     double cw[3], dmin[3], dw[3];
     double d, vol, dt, dem_dt;
     double ra, rg, rp, rv, rmax, cut2, skin2;
@@ -51,6 +52,7 @@ struct Scalars {
     int flag;                       // rebin this step
     int force;                      // rebin unconditionally at the next decision (first binning)
     long long rebuilds;
+    int list_overflow;              // longest row that did not fit the neighbour list (0 = none)
     double wall_center[2][3];
 };
 
@@ -66,6 +68,7 @@ struct Dev {
     R *dens_a, *gcx, *gcy, *gcz, *vs_p, *div_p, *press_p, *press_a, *mass, *kappa, *alpha, *drag_c;
     R *kn, *etan, *kt, *etat, *virial_p;
     int *ncount, *scount;
+    uint32_t *nlist;               // [kcap * n] neighbour entries, slot-interleaved
     int *cell_start;               // [3*ncell + 1]
     // rebinning scratch
     int *counts, *fill;            // [3*ncell + 1]
@@ -77,6 +80,7 @@ struct Dev {
     double *t_dbl;                 // [3n]
     // rigid bodies
     double *b_mass, *b_x, *b_v, *b_l, *b_q, *b_iinv, *b_f, *b_t, *b_w, *b_r;
+    double *blk_max;               // per-block maxima of the squared displacement
     Scalars *sc;
 };
 
@@ -90,8 +94,6 @@ template <class R> void launch_dem(const Geo &g, const Dev<R> &a, hipStream_t s)
 template <class R> void launch_move(const Geo &g, const Dev<R> &a, hipStream_t s);
 template <class R> void launch_step_end(const Geo &g, const Dev<R> &a, hipStream_t s);
 template <class R> void launch_virial(const Geo &g, const Dev<R> &a, hipStream_t s);
-template <class R> void launch_pairs(const Geo &g, const Dev<R> &a, int64_t *row_counts, const int64_t *ptr, int *idx,
-                                     hipStream_t s);
 template <class R> void launch_init_density(const Geo &g, const Dev<R> &a, hipStream_t s);
 
 }  // namespace dmph

@@ -6,6 +6,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <algorithm>
 #include <memory>
 #include <string>
 #include <type_traits>
@@ -285,6 +286,21 @@ class Engine final : public dmph_handle {
             else { err = "particle type outside 0..5"; return DMPH_E_FORMAT; }
         }
         g.fb = 0; g.fe = nf; g.sb = nf; g.se = nf + ns; g.wb = nf + ns; g.we = n;
+        if (n >= (1 << 26)) {   // neighbour entries pack the slot index in 26 bits
+            err = "more than 67,108,863 particles";
+            return DMPH_E_ARG;
+        }
+        {   // neighbour-list capacity: 1.6 x the lattice count within MaxRadius + 0.1 d, rounded up to 16
+            const double rc = (g.rmax + 0.1 * g.d) / g.d;
+            const int rr = (int)rc + 1, zr = g.dim == 2 ? 0 : rr;
+            int cnt = 0;
+            for (int ix = -rr; ix <= rr; ix++)
+                for (int iy = -rr; iy <= rr; iy++)
+                    for (int iz = -zr; iz <= zr; iz++)
+                        if ((double)(ix * ix + iy * iy + iz * iz) <= rc * rc) cnt++;
+            g.kcap = ((int)ceil(1.6 * cnt) + 15) / 16 * 16;
+            if (getenv("DMPH_NEIGHBOR_CAPACITY")) g.kcap = (atoi(getenv("DMPH_NEIGHBOR_CAPACITY")) + 7) / 8 * 8;
+        }
         g.rigid = o.debris == DMPH_DEBRIS_RIGID && gr->body != nullptr;
         TRY(allocate());
         TRY(upload(*gr));
@@ -311,6 +327,7 @@ class Engine final : public dmph_handle {
                        &a.kappa, &a.alpha, &a.drag_c, &a.kn, &a.etan, &a.kt, &a.etat, &a.virial_p};
         for (R **r : reals) TRY(alloc(*r, n));
         TRY(alloc(a.ncount, n)); TRY(alloc(a.scount, n));
+        TRY(alloc(a.nlist, (size_t)g.kcap * n));
         TRY(alloc(a.cell_start, nk)); TRY(alloc(a.counts, nk)); TRY(alloc(a.fill, nk));
         TRY(alloc(a.key_new, n)); TRY(alloc(a.order, n)); TRY(alloc(a.perm, n));
         if (g.sort_reference) { TRY(alloc(a.bkey, (size_t)g.p2)); TRY(alloc(a.bval, (size_t)g.p2)); }
@@ -318,6 +335,7 @@ class Engine final : public dmph_handle {
         TRY(alloc(a.t_int, 4 * n)); TRY(alloc(a.t_real, 8 * n));
         TRY(alloc(a.t_dbl, g.rigid ? 3 * n : 1)); TRY(alloc(a.r0, g.rigid ? 3 * n : 1));
         TRY(alloc(a.sc, 1));
+        TRY(alloc(a.blk_max, (size_t)(std::max(g.n, g.nb) + 255) / 256 + 1));
         return DMPH_OK;
     }
 
@@ -525,6 +543,19 @@ class Engine final : public dmph_handle {
     int sync() override {
         HCHK(hipSetDevice(device));
         HCHK(hipStreamSynchronize(stream));
+        return check_overflow();
+    }
+
+    int check_overflow() {
+        int ov = 0;
+        HCHK(hipMemcpy(&ov, &a.sc->list_overflow, sizeof ov, hipMemcpyDeviceToHost));
+        if (ov > 0) {
+            char b[200];
+            snprintf(b, sizeof b, "neighbour list capacity %d exceeded by a row of %d entries "
+                     "(set DMPH_NEIGHBOR_CAPACITY); results after that rebinning are invalid", g.kcap, ov);
+            err = b;
+            return DMPH_E_STATE;
+        }
         return DMPH_OK;
     }
 
@@ -679,32 +710,20 @@ class Engine final : public dmph_handle {
 
     int pairs(int64_t *nnz, int64_t *ptr, int *idx) override {
         HCHK(hipSetDevice(device));
+        TRY(sync());
         const int n = g.n;
-        int64_t *dcnt = nullptr, *dptr = nullptr;
-        int *didx = nullptr;
-        HCHK(hipMalloc(&dcnt, sizeof(int64_t) * n));
-        launch_pairs<R>(g, a, dcnt, nullptr, nullptr, stream);
-        std::vector<int64_t> cnt(n);
-        hipError_t e = hipMemcpyAsync(cnt.data(), dcnt, sizeof(int64_t) * n, hipMemcpyDeviceToHost, stream);
-        if (e == hipSuccess) e = hipStreamSynchronize(stream);
-        (void)hipFree(dcnt);
-        if (e != hipSuccess) { err = hipGetErrorString(e); return DMPH_E_DEVICE; }
+        std::vector<int> cnt;
+        TRY(download(cnt, a.ncount, n));
         std::vector<int64_t> pt(n + 1, 0);
         for (int i = 0; i < n; i++) pt[i + 1] = pt[i] + cnt[i];
         if (nnz) *nnz = pt[n];
         if (ptr) memcpy(ptr, pt.data(), sizeof(int64_t) * (n + 1));
         if (!idx) return DMPH_OK;
-        HCHK(hipMalloc(&dptr, sizeof(int64_t) * (n + 1)));
-        e = hipMalloc(&didx, sizeof(int) * (pt[n] > 0 ? pt[n] : 1));
-        if (e == hipSuccess) e = hipMemcpyAsync(dptr, pt.data(), sizeof(int64_t) * (n + 1), hipMemcpyHostToDevice, stream);
-        if (e == hipSuccess) {
-            launch_pairs<R>(g, a, nullptr, dptr, didx, stream);
-            e = hipMemcpyAsync(idx, didx, sizeof(int) * pt[n], hipMemcpyDeviceToHost, stream);
-        }
-        if (e == hipSuccess) e = hipStreamSynchronize(stream);
-        (void)hipFree(dptr);
-        if (didx) (void)hipFree(didx);
-        if (e != hipSuccess) { err = hipGetErrorString(e); return DMPH_E_DEVICE; }
+        std::vector<uint32_t> list;
+        TRY(download(list, a.nlist, (size_t)g.kcap * n));
+        for (int i = 0; i < n; i++)
+            for (int k = 0; k < cnt[i]; k++)
+                idx[pt[i] + k] = (int)(list[((size_t)(k >> 3) * n + i) * 8 + (k & 7)] >> 6);
         return DMPH_OK;
     }
 
