@@ -78,6 +78,23 @@ def cpu_baseline(sc, params, budget_s: float = 20.0):
                       % (sc.n, steps, el)}
 
 
+def copy_probe(mib: int) -> None:
+    """One device-to-device copy of `mib` MiB through the HIP runtime: the copy
+    kernel's FETCH_SIZE / WRITE_SIZE under rocprof calibrate the counters
+    against a known byte count."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    n = mib << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(n)) == 0 and hip.hipMalloc(C.byref(b), C.c_size_t(n)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(n)) == 0
+    assert hip.hipMemcpy(b, a, C.c_size_t(n), 3) == 0   # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print("bench: copy probe of %d MiB done" % mib, file=sys.stderr, flush=True)
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -87,6 +104,8 @@ def main(argv=None) -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="also run one device-to-device copy of MIB MiB (counter calibration)")
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print("bench.py measures one GPU: the DEM-MPH step here runs on a single MI355X "
@@ -94,6 +113,8 @@ def main(argv=None) -> int:
         return 2
 
     import dem_mph_amd as D
+    if a.copy_probe:
+        copy_probe(a.copy_probe)
     sc, params = build_workload(a.seed)
     prec = D.PREC_FP32 if a.precision == "fp32" else D.PREC_FP64
     print("bench: building solver for %d particles (%s)" % (sc.n, a.precision), file=sys.stderr, flush=True)

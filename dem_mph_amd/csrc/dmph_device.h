@@ -34,9 +34,11 @@ struct Geo {
     int p2, p2_log;     // bitonic size (power of two > n) and its log2
     int compensated;    // Kahan sums (FP32)
     int kcap;           // neighbour-list capacity per particle
+    int ccap;           // DEM contact-list capacity per particle
     double cw[3], dmin[3], dw[3];
     double d, vol, dt, dem_dt;
     double ra, rg, rp, rv, rmax, cut2, skin2;
+    double near2, con2;  // squared cutoffs of the FP32 near segment and of the DEM contact list
     double swa, swg, swp, swv, n0a, n0p, r2g, cofk;
     double cofa[kNType], dens[kNType], bulk[kNType], bvisc[kNType], shear[kNType];
     double young[kNType], poisson[kNType], logrest[kNType], fric[kNType];
@@ -69,6 +71,9 @@ struct Dev {
     R *kn, *etan, *kt, *etat, *virial_p;
     int *ncount, *scount;
     uint32_t *nlist;               // [kcap * n] neighbour entries, slot-interleaved
+    int *nnear;                    // FP32: entries in the near segment of each row
+    uint32_t *clist;               // [ccap * (n - sb)] DEM contact entries of solid and wall slots
+    int *ncontact;                 // [n - sb]
     int *cell_start;               // [3*ncell + 1]
     // rebinning scratch
     int *counts, *fill;            // [3*ncell + 1]
@@ -81,6 +86,7 @@ struct Dev {
     // rigid bodies
     double *b_mass, *b_x, *b_v, *b_l, *b_q, *b_iinv, *b_f, *b_t, *b_w, *b_r;
     double *blk_max;               // per-block maxima of the squared displacement
+    float4 *p4, *v4;               // FP32 sweeps: packed (position, type) and (velocity, PressureP)
     Scalars *sc;
 };
 

@@ -300,6 +300,14 @@ class Engine final : public dmph_handle {
                         if ((double)(ix * ix + iy * iy + iz * iz) <= rc * rc) cnt++;
             g.kcap = ((int)ceil(1.6 * cnt) + 15) / 16 * 16;
             if (getenv("DMPH_NEIGHBOR_CAPACITY")) g.kcap = (atoi(getenv("DMPH_NEIGHBOR_CAPACITY")) + 7) / 8 * 8;
+            g.ccap = 32;
+            // near segment (FP32): every pass but the viscosity and surface-tension
+            // ones stays within RadiusP; the 0.1 d skin keeps it conservative
+            const double rn = (g.st_active ? std::max(g.rp, g.ra) : g.rp) + 0.1 * g.d;
+            g.near2 = rn * rn;
+            // contact list: within d now means within d + 0.1 d at rebinning; a further
+            // 0.15 d covers the solids' motion over the DEM sub-steps of one step
+            g.con2 = (1.25 * g.d) * (1.25 * g.d);
         }
         g.rigid = o.debris == DMPH_DEBRIS_RIGID && gr->body != nullptr;
         TRY(allocate());
@@ -328,6 +336,9 @@ class Engine final : public dmph_handle {
         for (R **r : reals) TRY(alloc(*r, n));
         TRY(alloc(a.ncount, n)); TRY(alloc(a.scount, n));
         TRY(alloc(a.nlist, (size_t)g.kcap * n));
+        TRY(alloc(a.nnear, n));
+        TRY(alloc(a.clist, (size_t)g.ccap * (n - g.sb)));
+        TRY(alloc(a.ncontact, n - g.sb));
         TRY(alloc(a.cell_start, nk)); TRY(alloc(a.counts, nk)); TRY(alloc(a.fill, nk));
         TRY(alloc(a.key_new, n)); TRY(alloc(a.order, n)); TRY(alloc(a.perm, n));
         if (g.sort_reference) { TRY(alloc(a.bkey, (size_t)g.p2)); TRY(alloc(a.bval, (size_t)g.p2)); }
@@ -336,6 +347,7 @@ class Engine final : public dmph_handle {
         TRY(alloc(a.t_dbl, g.rigid ? 3 * n : 1)); TRY(alloc(a.r0, g.rigid ? 3 * n : 1));
         TRY(alloc(a.sc, 1));
         TRY(alloc(a.blk_max, (size_t)(std::max(g.n, g.nb) + 255) / 256 + 1));
+        if (!std::is_same<R, double>::value) { TRY(alloc(a.p4, n)); TRY(alloc(a.v4, n)); }
         return DMPH_OK;
     }
 
