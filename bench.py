@@ -95,13 +95,48 @@ def copy_probe(mib: int) -> None:
     print("bench: copy probe of %d MiB done" % mib, file=sys.stderr, flush=True)
 
 
+TRAFFIC_JSON = os.path.join(ROOT, "profiles", "dam_break_c2_fp32_traffic.json")
+
+
+def kernel_traffic(kernel_name: str):
+    """HBM bytes per launch of a kernel from the committed rocprofv3 summary
+    (FETCH_SIZE + WRITE_SIZE, calibrated), or None."""
+    try:
+        js = json.load(open(TRAFFIC_JSON))
+    except Exception:
+        return None
+    for name, v in js.get("traffic", {}).items():
+        if kernel_name in name:
+            return v["bytes"]
+    return None
+
+
+def time_variant(D, sc, params, prec, compensated, steps, warmup):
+    s = D.Solver(D.Params.from_dict(params), sc, precision=prec, rigid=True, compensated=compensated)
+    info = s.info()
+    s.step(warmup)
+    s.sync()
+    reb0 = s.info().rebuilds
+    s.set_timing(True)
+    t0 = time.perf_counter()
+    s.step(steps)
+    s.sync()
+    el = time.perf_counter() - t0
+    tm = s.timing()
+    reb = s.info().rebuilds
+    s.set_timing(False)
+    s.close()
+    return {"seconds": el, "timing": tm, "info": info, "rebuilds": int(reb - reb0)}
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--precision", choices=("fp32", "fp64"), default="fp32")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain (uncompensated) FP32 variant")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -117,30 +152,26 @@ def main(argv=None) -> int:
         copy_probe(a.copy_probe)
     sc, params = build_workload(a.seed)
     prec = D.PREC_FP32 if a.precision == "fp32" else D.PREC_FP64
-    print("bench: building solver for %d particles (%s)" % (sc.n, a.precision), file=sys.stderr, flush=True)
-    s = D.Solver(D.Params.from_dict(params), sc, precision=prec, rigid=True)
-    info = s.info()
-    s.step(a.warmup)
-    s.sync()
-    s.set_timing(True)
-    t0 = time.perf_counter()
-    s.step(a.steps)
-    s.sync()
-    el = time.perf_counter() - t0
-    tm = s.timing()
-    reb0 = info.rebuilds
-    reb = s.info().rebuilds
-    s.set_timing(False)
     n = sc.n
-    pss = n * a.steps / el
     dt = params["Dt"]
+    print("bench: %d particles, %s, mixed precision (Kahan sums, FP64 rigid bodies)" % (n, a.precision),
+          file=sys.stderr, flush=True)
+    mixed = time_variant(D, sc, params, prec, True, a.steps, a.warmup)
+    variants = {"mixed": mixed}
+    if not a.no_plain and prec == D.PREC_FP32:
+        print("bench: plain FP32 sums", file=sys.stderr, flush=True)
+        variants["plain"] = time_variant(D, sc, params, prec, False, a.steps, a.warmup)
+    info = mixed["info"]
+    el = mixed["seconds"]
+    pss = n * a.steps / el
+    groups = {k: v for k, v in mixed["timing"].items() if v[1] > 0}
     # dominant kernel: the one with the largest total device time
-    groups = {k: v for k, v in tm.items() if v[1] > 0}
     dom = max(("sweep1", "sweep2", "dem"), key=lambda k: groups.get(k, (0, 0))[0])
     ms, launches = groups[dom]
     units = {"sweep1": n, "sweep2": info.solid_end - info.fluid_begin, "dem": info.wall_end - info.solid_begin}[dom]
     avg_s = ms / 1e3 / launches
     achieved = ALGO_BYTES[dom] * units / avg_s / 1e9
+    kname = {"sweep1": "k_sweep1<float", "sweep2": "k_sweep2<float", "dem": "k_dem<float"}[dom]
     out = {
         "metric": METRIC,
         "value": pss,
@@ -156,17 +187,19 @@ def main(argv=None) -> int:
         "data": "synthetic (seeded dam-break scene, jitter +-0.01 d)",
         "config": {"workload": "dam_break_c2: 3D tank 0.8x0.4x0.6 m, d=0.004 m, %d particles "
                                "(562500 fluid, 64 rigid debris bodies of 10^3, 340908 wall), dt=2e-5 s" % n,
-                   "particles": n, "bodies": info.nbodies, "precision": a.precision,
-                   "rigid_body_math": "f64", "sort": "canonical", "rebuild": "skin 0.05 d"},
+                   "particles": n, "bodies": info.nbodies, "precision": "mixed: FP32 pairs, Kahan sums, "
+                   "FP64 rigid bodies and time" if prec == D.PREC_FP32 else "fp64",
+                   "sort": "canonical", "rebuild": "skin 0.05 d"},
         "sim_s_per_wall_s": dt * a.steps / el,
-        "rebuilds_in_timed_region": int(reb - reb0),
+        "rebuilds_in_timed_region": mixed["rebuilds"],
         "kernel_ms_per_step": {k: v[0] / a.steps for k, v in groups.items()},
+        "variants": {k: {"particle_steps_per_s": n * a.steps / v["seconds"], "ms_per_step": v["seconds"] / a.steps * 1e3,
+                         "sim_s_per_wall_s": dt * a.steps / v["seconds"]} for k, v in variants.items()},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": kernel_traffic(kname),
                      "algorithmic_bytes_per_unit": ALGO_BYTES[dom], "units_per_launch": units,
                      "avg_launch_ms": avg_s * 1e3},
     }
-    s.close()
     if not a.no_cpu_baseline:
         print("bench: timing the CPU baseline (bounded sample)", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(sc, params, a.cpu_budget)

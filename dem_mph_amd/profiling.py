@@ -1,0 +1,103 @@
+"""Summaries of rocprofv3 output databases (SQLite, ``*_results.db``).
+
+    python -m dem_mph_amd.profiling --trace T.db [--fetch F.db --write W.db --copy-mib 256]
+                                    [--title ...] [--command ...] --out profiles/<name>.md [--json ...]
+
+- kernel statistics from a ``--kernel-trace --stats`` run (calls, average and
+  total duration per kernel);
+- HBM traffic per dispatch from two ``--pmc`` runs, FETCH_SIZE and WRITE_SIZE
+  (KiB per dispatch), each scaled by what the same counter reads for a plain
+  device-to-device copy of known size (``bench.py --copy-probe MIB``): the copy
+  kernel is the ``__amd_rocclr_copyBuffer`` dispatch with the largest count.
+"""
+from __future__ import annotations
+
+import argparse
+import collections
+import json
+import sqlite3
+from typing import Dict, List, Tuple
+
+
+def kernel_stats(db: str) -> List[Tuple[str, int, float, float]]:
+    """(name, calls, average us, total us) per kernel, by total time."""
+    c = sqlite3.connect(db)
+    rows = []
+    for name, n, tot in c.execute("select name, count(*), sum(duration) from kernels group by name"):
+        rows.append((name, int(n), tot / n / 1e3, tot / 1e3))
+    rows.sort(key=lambda r: -r[3])
+    return rows
+
+
+def pmc_per_dispatch(db: str, counter: str) -> Dict[str, List[float]]:
+    """Counter value per dispatch (summed over hardware instances), grouped by kernel name."""
+    c = sqlite3.connect(db)
+    per = collections.defaultdict(float)
+    names = {}
+    for name, v, did in c.execute("select name, counter_value, dispatch_id from pmc_events where counter_name = ?",
+                                  (counter,)):
+        per[did] += v
+        names[did] = name
+    out = collections.defaultdict(list)
+    for did, v in per.items():
+        out[names[did]].append(v)
+    return out
+
+
+def traffic(fetch_db: str, write_db: str, copy_mib: int) -> Tuple[Dict[str, Dict[str, float]], Dict[str, float]]:
+    """Calibrated HBM bytes per dispatch per kernel, and the calibration factors."""
+    f = pmc_per_dispatch(fetch_db, "FETCH_SIZE")
+    w = pmc_per_dispatch(write_db, "WRITE_SIZE")
+    known_kib = copy_mib * 1024.0
+    cf = known_kib / max(f.get("__amd_rocclr_copyBuffer", [known_kib]))
+    cw = known_kib / max(w.get("__amd_rocclr_copyBuffer", [known_kib]))
+    out = {}
+    for name in set(f) | set(w):
+        fr = sum(f.get(name, [0.0])) / max(len(f.get(name, [1])), 1)
+        wr = sum(w.get(name, [0.0])) / max(len(w.get(name, [1])), 1)
+        out[name] = {"fetch_kib_raw": fr, "write_kib_raw": wr, "read_bytes": fr * 1024 * cf,
+                     "write_bytes": wr * 1024 * cw, "bytes": fr * 1024 * cf + wr * 1024 * cw}
+    return out, {"fetch": cf, "write": cw}
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--trace", required=True)
+    ap.add_argument("--fetch")
+    ap.add_argument("--write")
+    ap.add_argument("--copy-mib", type=int, default=256)
+    ap.add_argument("--title", default="rocprofv3 summary")
+    ap.add_argument("--command", default="")
+    ap.add_argument("--out", required=True)
+    ap.add_argument("--json")
+    ap.add_argument("--top", type=int, default=14)
+    a = ap.parse_args(argv)
+    st = kernel_stats(a.trace)
+    lines = ["# " + a.title, ""]
+    if a.command:
+        lines += ["Command:", "", "```", a.command, "```", ""]
+    lines += ["| kernel | calls | avg us | total us |", "|---|---|---|---|"]
+    for name, n, avg, tot in st[: a.top]:
+        lines.append("| `%s` | %d | %.2f | %.1f |" % (name, n, avg, tot))
+    js = {"kernels": {n: {"calls": c, "avg_us": av, "total_us": t} for n, c, av, t in st}}
+    if a.fetch and a.write:
+        tr, fac = traffic(a.fetch, a.write, a.copy_mib)
+        lines += ["", "HBM traffic per dispatch (FETCH_SIZE x %.3f, WRITE_SIZE x %.3f; factors from a %d MiB "
+                      "device-to-device copy):" % (fac["fetch"], fac["write"], a.copy_mib), "",
+                  "| kernel | read MB | write MB | total MB |", "|---|---|---|---|"]
+        for name, v in sorted(tr.items(), key=lambda kv: -kv[1]["bytes"]):
+            if v["bytes"] < 1e6:
+                continue
+            lines.append("| `%s` | %.1f | %.1f | %.1f |" % (name, v["read_bytes"] / 1e6, v["write_bytes"] / 1e6,
+                                                          v["bytes"] / 1e6))
+        js["traffic"] = tr
+        js["calibration"] = fac
+    open(a.out, "w").write("\n".join(lines) + "\n")
+    if a.json:
+        json.dump(js, open(a.json, "w"), indent=1, sort_keys=True)
+    print("\n".join(lines))
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())
