@@ -102,4 +102,8 @@ template <class R> void launch_step_end(const Geo &g, const Dev<R> &a, hipStream
 template <class R> void launch_virial(const Geo &g, const Dev<R> &a, hipStream_t s);
 template <class R> void launch_init_density(const Geo &g, const Dev<R> &a, hipStream_t s);
 
+// FP32 sweeps (dmph_sweep32.inc, in the FP32 object only)
+void launch_sweep1_f32(const Geo &g, const Dev<float> &a, hipStream_t s, int init);
+void launch_sweep2_f32(const Geo &g, const Dev<float> &a, hipStream_t s);
+
 }  // namespace dmph

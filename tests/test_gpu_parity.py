@@ -107,9 +107,9 @@ def test_fp64_canonical_order_matches_oracle(name):
     assert max(errs.values()) <= 1e-9, errs
 
 
-def _fp32_vs_oracle(name, steps):
+def _fp32_vs_oracle(name, steps, compensated=True):
     g = G.load(name)
-    s = _solver(g, precision=D.PREC_FP32, sort_mode=D.SORT_CANONICAL)
+    s = _solver(g, precision=D.PREC_FP32, sort_mode=D.SORT_CANONICAL, compensated=compensated)
     o = P.Oracle(P.params_from_dict(G.params(g)), G.scene(g), sort_mode=P.SORT_CANONICAL)
     # binning happens in FP64 on the first call: identical keys and permutation
     np.testing.assert_array_equal(s.field("key", D.ORDER_SLOT), o.get("key"))
@@ -118,15 +118,16 @@ def _fp32_vs_oracle(name, steps):
     return g, s, o
 
 
-@pytest.mark.parametrize("name", ["mix3d", "st3d", "mix2d"])
-def test_fp32_one_step_within_stated_tolerance(name):
-    """FP32 pairs on cell-relative positions, one step.
+@pytest.mark.parametrize("compensated", [True, False], ids=["fp64sums", "fp32sums"])
+@pytest.mark.parametrize("name", ["mix3d", "st3d", "mix2d", "st2d", "wall2d"])
+def test_fp32_one_step_within_stated_tolerance(name, compensated):
+    """FP32 pairs on cell-relative positions, one step, with FP64 or FP32 sums.
     Stated tolerances (norm-relative, ||x - x64|| / ||x64|| over the field):
       number density sum_j w_p (= VolStrainP + N0p)     1e-6 (per particle: 1e-5 relative)
       fluid acceleration F/m, solid contact force        1e-4
       positions                                           8 ulp of a cell-relative FP32 offset
                                                           (2^-21 x CellWidth: 3e-8 m for 0.065 m cells)"""
-    g, s, o = _fp32_vs_oracle(name, 1)
+    g, s, o = _fp32_vs_oracle(name, 1, compensated)
     n0p = o.scalars()["n0p"]
     nd_ref = o.field("vol_strain_p") + n0p
     nd = s.field("vol_strain_p") + n0p
@@ -137,12 +138,13 @@ def test_fp32_one_step_within_stated_tolerance(name):
     fl = typ < 2
     acc_err = np.linalg.norm((F[fl] - Fr[fl]) / m[fl, None]) / np.linalg.norm(Fr[fl] / m[fl, None])
     sol = (typ >= 2) & (typ < 4)
-    sf_err = np.linalg.norm(F[sol] - Fr[sol]) / np.linalg.norm(Fr[sol])
+    sf_err = np.linalg.norm(F[sol] - Fr[sol]) / np.linalg.norm(Fr[sol]) if sol.any() else 0.0
     dx = s.field("pos") - o.field("pos")
     L = g["domain_max"] - g["domain_min"]
     dx = dx - L * np.round(dx / L)
-    _report(name, {"density_per_particle": per, "fluid_accel": acc_err, "solid_force": sf_err,
-                   "pos_abs": np.abs(dx).max()})
+    _report("%s-%s" % (name, "fp64sums" if compensated else "fp32sums"),
+            {"density_per_particle": per, "fluid_accel": acc_err, "solid_force": sf_err,
+             "pos_abs": np.abs(dx).max()})
     assert per <= 1e-5
     assert acc_err <= 1e-4
     assert sf_err <= 1e-4
