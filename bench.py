@@ -99,16 +99,15 @@ TRAFFIC_JSON = os.path.join(ROOT, "profiles", "dam_break_c2_fp32_traffic.json")
 
 
 def kernel_traffic(kernel_name: str):
-    """HBM bytes per launch of a kernel from the committed rocprofv3 summary
-    (FETCH_SIZE + WRITE_SIZE, calibrated), or None."""
+    """HBM bytes per launch of a kernel group (every kernel whose name contains
+    `kernel_name`, summed) from the committed rocprofv3 summary (FETCH_SIZE +
+    WRITE_SIZE, calibrated by a plain copy), or None."""
     try:
         js = json.load(open(TRAFFIC_JSON))
     except Exception:
         return None
-    for name, v in js.get("traffic", {}).items():
-        if kernel_name in name:
-            return v["bytes"]
-    return None
+    hits = [v["bytes"] for name, v in js.get("traffic", {}).items() if kernel_name in name]
+    return sum(hits) if hits else None
 
 
 def time_variant(D, sc, params, prec, compensated, steps, warmup):
@@ -136,7 +135,7 @@ def main(argv=None) -> int:
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--precision", choices=("fp32", "fp64"), default="fp32")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-plain", action="store_true", help="skip the plain (uncompensated) FP32 variant")
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain variant (FP32 sums)")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -154,7 +153,7 @@ def main(argv=None) -> int:
     prec = D.PREC_FP32 if a.precision == "fp32" else D.PREC_FP64
     n = sc.n
     dt = params["Dt"]
-    print("bench: %d particles, %s, mixed precision (Kahan sums, FP64 rigid bodies)" % (n, a.precision),
+    print("bench: %d particles, %s, mixed precision (FP64 sums, FP64 rigid bodies)" % (n, a.precision),
           file=sys.stderr, flush=True)
     mixed = time_variant(D, sc, params, prec, True, a.steps, a.warmup)
     variants = {"mixed": mixed}
@@ -171,7 +170,7 @@ def main(argv=None) -> int:
     units = {"sweep1": n, "sweep2": info.solid_end - info.fluid_begin, "dem": info.wall_end - info.solid_begin}[dom]
     avg_s = ms / 1e3 / launches
     achieved = ALGO_BYTES[dom] * units / avg_s / 1e9
-    kname = {"sweep1": "k_sweep1<float", "sweep2": "k_sweep2<float", "dem": "k_dem<float"}[dom]
+    kname = {"sweep1": "k_sweep1f<", "sweep2": "k_sweep2f<", "dem": "k_dem<float"}[dom]
     out = {
         "metric": METRIC,
         "value": pss,
@@ -187,7 +186,7 @@ def main(argv=None) -> int:
         "data": "synthetic (seeded dam-break scene, jitter +-0.01 d)",
         "config": {"workload": "dam_break_c2: 3D tank 0.8x0.4x0.6 m, d=0.004 m, %d particles "
                                "(562500 fluid, 64 rigid debris bodies of 10^3, 340908 wall), dt=2e-5 s" % n,
-                   "particles": n, "bodies": info.nbodies, "precision": "mixed: FP32 pairs, Kahan sums, "
+                   "particles": n, "bodies": info.nbodies, "precision": "mixed: FP32 pairs, FP64 sums, "
                    "FP64 rigid bodies and time" if prec == D.PREC_FP32 else "fp64",
                    "sort": "canonical", "rebuild": "skin 0.05 d"},
         "sim_s_per_wall_s": dt * a.steps / el,

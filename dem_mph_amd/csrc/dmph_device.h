@@ -32,7 +32,7 @@ struct Geo {
     int sort_reference; // within-cell order from the reference's bitonic network
     int rebuild_every;  // rebin every step instead of the skin test
     int p2, p2_log;     // bitonic size (power of two > n) and its log2
-    int compensated;    // Kahan sums (FP32)
+    int compensated;    // FP32 mode: FP64 accumulators for the pair sums
     int kcap;           // neighbour-list capacity per particle
     int ccap;           // DEM contact-list capacity per particle
     double cw[3], dmin[3], dw[3];

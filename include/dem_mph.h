@@ -71,7 +71,7 @@ typedef struct dmph_options {
     int rebuild;        /* DMPH_REBUILD_*: reference 0.05 d skin test, or rebin every step */
     int debris;         /* DMPH_DEBRIS_*: loose DEM grains (reference) or rigid bodies */
     int device;         /* HIP device ordinal */
-    int compensated;    /* FP32 only: Kahan-compensated force/density sums */
+    int compensated;    /* FP32 only: pair sums carried in FP64 accumulators (1, default) or FP32 (0) */
 } dmph_options;
 
 void dmph_default_options(dmph_options *opt);
