@@ -107,7 +107,7 @@ class Grid(C.Structure):
 
 class Options(C.Structure):
     _fields_ = [("dim", C.c_int), ("precision", C.c_int), ("sort_mode", C.c_int), ("rebuild", C.c_int),
-                ("debris", C.c_int), ("device", C.c_int), ("compensated", C.c_int)]
+                ("debris", C.c_int), ("device", C.c_int), ("compensated", C.c_int), ("graph", C.c_int)]
 
 
 class Info(C.Structure):
@@ -190,7 +190,7 @@ class Solver:
 
     def __init__(self, params: Params, scene, precision: int = PREC_FP64, sort_mode: int = SORT_CANONICAL,
                  rebuild: int = REBUILD_SKIN, rigid: bool = False, dim: Optional[int] = None, device: int = 0,
-                 compensated: bool = True):
+                 compensated: bool = True, graph: bool = True):
         self._arrays = []
         t = np.ascontiguousarray(scene.type, np.int32)
         pos = np.ascontiguousarray(scene.pos, np.float64)
@@ -215,7 +215,7 @@ class Solver:
         self.options = default_options(dim=dim if dim else (scene.dim if scene.dim in (2, 3) else 0),
                                        precision=precision, sort_mode=sort_mode, rebuild=rebuild,
                                        debris=DEBRIS_RIGID if rigid else DEBRIS_PARTICLES, device=device,
-                                       compensated=1 if compensated else 0)
+                                       compensated=1 if compensated else 0, graph=1 if graph else 0)
         self.params = params
         h = C.c_void_p()
         rc = lib().dmph_create(C.byref(params), C.byref(self.options), C.byref(g), C.byref(h))

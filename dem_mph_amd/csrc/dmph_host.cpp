@@ -217,6 +217,8 @@ class Engine final : public dmph_handle {
 
     ~Engine() override {
         if (stream) (void)hipStreamSynchronize(stream);
+        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+        if (graph) (void)hipGraphDestroy(graph);
         for (auto &p_ : pending) { ev_pool.push_back(p_.second.first); ev_pool.push_back(p_.second.second); }
         for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
         for (void *q : allocs) (void)hipFree(q);
@@ -533,20 +535,52 @@ class Engine final : public dmph_handle {
         return DMPH_OK;
     }
 
+    // One pass of the loop body: every launch of a step, in order.  No host
+    // synchronisation and no memset: the rebin decision is a device flag that
+    // the rebin kernels read, so the same sequence serves every step.
+    void enqueue_step() {
+        timed(DMPH_TIMING_PREDICT, [&] { launch_predict<R>(g, a, stream); });
+        timed(DMPH_TIMING_REBIN, [&] { launch_rebin<R>(g, a, stream); });
+        timed(DMPH_TIMING_SWEEP1, [&] { launch_sweep1<R>(g, a, stream); });
+        timed(DMPH_TIMING_SWEEP2, [&] { launch_sweep2<R>(g, a, stream); });
+        timed(DMPH_TIMING_ACCEL, [&] { launch_accel<R>(g, a, stream); });
+        for (int s = 0; s < g.substeps; s++) {
+            timed(DMPH_TIMING_DEM, [&] { launch_dem<R>(g, a, stream); });
+            timed(DMPH_TIMING_MOVE, [&] { launch_move<R>(g, a, stream); });
+        }
+        timed(DMPH_TIMING_END, [&] { launch_step_end<R>(g, a, stream); });
+    }
+
+    // The step as a HIP graph, captured on first use and replayed once per step.
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+
+    int capture_step() {
+        HCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        enqueue_step();
+        hipGraph_t gr = nullptr;
+        const hipError_t e = hipStreamEndCapture(stream, &gr);
+        if (e != hipSuccess || !gr) {
+            err = std::string("hipStreamEndCapture: ") + hipGetErrorString(e);
+            return DMPH_E_DEVICE;
+        }
+        graph = gr;
+        HCHK(hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0));
+        return DMPH_OK;
+    }
+
     int step(int nsteps) override {
         HCHK(hipSetDevice(device));
+        // per-group event timing needs the launches one by one: eager then
+        const bool use_graph = o.graph && !timing;
+        if (use_graph && !graph_exec && nsteps > 0) TRY(capture_step());
         for (int k = 0; k < nsteps; k++) {
-            timed(DMPH_TIMING_PREDICT, [&] { launch_predict<R>(g, a, stream); });
-            timed(DMPH_TIMING_REBIN, [&] { launch_rebin<R>(g, a, stream); });
-            timed(DMPH_TIMING_SWEEP1, [&] { launch_sweep1<R>(g, a, stream); });
-            timed(DMPH_TIMING_SWEEP2, [&] { launch_sweep2<R>(g, a, stream); });
-            timed(DMPH_TIMING_ACCEL, [&] { launch_accel<R>(g, a, stream); });
-            for (int s = 0; s < g.substeps; s++) {
-                timed(DMPH_TIMING_DEM, [&] { launch_dem<R>(g, a, stream); });
-                timed(DMPH_TIMING_MOVE, [&] { launch_move<R>(g, a, stream); });
+            if (use_graph) {
+                HCHK(hipGraphLaunch(graph_exec, stream));
+            } else {
+                enqueue_step();
+                if (timing && pending.size() > 4096) TRY(resolve_timing());
             }
-            timed(DMPH_TIMING_END, [&] { launch_step_end<R>(g, a, stream); });
-            if (timing && pending.size() > 4096) TRY(resolve_timing());
         }
         HCHK(hipGetLastError());
         return DMPH_OK;
@@ -796,6 +830,7 @@ void dmph_default_options(dmph_options *opt) {
     opt->debris = DMPH_DEBRIS_PARTICLES;
     opt->device = 0;
     opt->compensated = 1;
+    opt->graph = 1;
 }
 
 int dmph_create(const dmph_params *p, const dmph_options *opt, const dmph_grid *g, dmph_handle **out) {

@@ -72,6 +72,7 @@ typedef struct dmph_options {
     int debris;         /* DMPH_DEBRIS_*: loose DEM grains (reference) or rigid bodies */
     int device;         /* HIP device ordinal */
     int compensated;    /* FP32 only: pair sums carried in FP64 accumulators (1, default) or FP32 (0) */
+    int graph;          /* 1 (default): each step replays one captured hipGraph; 0: launches one by one */
 } dmph_options;
 
 void dmph_default_options(dmph_options *opt);
@@ -96,7 +97,9 @@ int dmph_create(const dmph_params *p, const dmph_options *opt, const dmph_grid *
 /* `nsteps` passes of the main loop body (source/DEM-MPH.cpp:450-579): fluid
  * convection, walls, periodic wrap, rebuild on the 0.05 d skin test, the MPH
  * passes, drag (twice), buoyancy, gravity, acceleration and the DEM sub-steps.
- * Asynchronous on the handle's stream; no host synchronisation per step. */
+ * Asynchronous on the handle's stream; no host synchronisation per step.  With
+ * options.graph the step is captured once as a hipGraph and replayed (while
+ * device timing is enabled the launches are issued one by one). */
 int dmph_step(dmph_handle *h, int nsteps);
 
 /* Wait for queued work; returns a pending device error if any. */
