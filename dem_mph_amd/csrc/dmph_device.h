@@ -56,6 +56,7 @@ struct Scalars {
     long long rebuilds;
     int list_overflow;              // longest row that did not fit the neighbour list (0 = none)
     double wall_center[2][3];
+    unsigned int ticket[4];         // blocks finished, per kernel that ends in a last-block phase
 };
 
 template <class R>
@@ -98,7 +99,6 @@ template <class R> void launch_sweep2(const Geo &g, const Dev<R> &a, hipStream_t
 template <class R> void launch_accel(const Geo &g, const Dev<R> &a, hipStream_t s);
 template <class R> void launch_dem(const Geo &g, const Dev<R> &a, hipStream_t s);
 template <class R> void launch_move(const Geo &g, const Dev<R> &a, hipStream_t s);
-template <class R> void launch_step_end(const Geo &g, const Dev<R> &a, hipStream_t s);
 template <class R> void launch_virial(const Geo &g, const Dev<R> &a, hipStream_t s);
 template <class R> void launch_init_density(const Geo &g, const Dev<R> &a, hipStream_t s);
 

@@ -548,7 +548,6 @@ class Engine final : public dmph_handle {
             timed(DMPH_TIMING_DEM, [&] { launch_dem<R>(g, a, stream); });
             timed(DMPH_TIMING_MOVE, [&] { launch_move<R>(g, a, stream); });
         }
-        timed(DMPH_TIMING_END, [&] { launch_step_end<R>(g, a, stream); });
     }
 
     // The step as a HIP graph, captured on first use and replayed once per step.

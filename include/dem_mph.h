@@ -168,14 +168,14 @@ int dmph_write_vtk(dmph_handle *h, const char *path);
  * reference counterpart; the reference's clock() buckets, source/DEM-MPH.cpp:345,
  * measure host CPU time).  Groups: DMPH_TIMING_*. */
 enum {
-    DMPH_TIMING_PREDICT = 0,   /* convection, walls, wrap, skin test */
+    DMPH_TIMING_PREDICT = 0,   /* convection, walls, wrap, skin test, rebin decision, time advance */
     DMPH_TIMING_REBIN = 1,     /* keys, scan, sort, reorder (early exit on steps without rebinning) */
     DMPH_TIMING_SWEEP1 = 2,    /* densities, divergence, pressures */
     DMPH_TIMING_SWEEP2 = 3,    /* drag, pressure, surface tension, viscosity, buoyancy, gravity forces */
     DMPH_TIMING_ACCEL = 4,     /* fluid velocity update */
     DMPH_TIMING_DEM = 5,       /* DEM coefficients and contact forces (per sub-step) */
     DMPH_TIMING_MOVE = 6,      /* solid update, rigid-body reduction/integration/re-posing (per sub-step) */
-    DMPH_TIMING_END = 7,       /* time accumulator */
+    DMPH_TIMING_END = 7,       /* unused: the time accumulator advances in the predict kernel */
     DMPH_TIMING_GROUPS = 8
 };
 int dmph_set_timing(dmph_handle *h, int enable);
