@@ -51,7 +51,8 @@ def build_workload(seed: int = 1):
 
 def cpu_baseline(sc, params, budget_s: float = 20.0):
     """The FP64 oracle (OpenMP, all host threads) on the same scene: set-up is
-    not timed; steps are run until about `budget_s` seconds of wall time."""
+    not timed; steps are run until about `budget_s` seconds of wall time.
+    Returns the baseline record and the oracle, left at that step."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as P
     if not os.path.exists(P.LIB_PATH):
@@ -72,10 +73,59 @@ def cpu_baseline(sc, params, budget_s: float = 20.0):
         el = time.perf_counter() - t0
         if el >= budget_s or steps >= 50:
             break
-    o.close()
-    return {"value": sc.n * steps / el, "unit": "particle-steps/s", "cores": threads, "kind": "port",
-            "sample": "FP64 oracle, rigid debris, same %d-particle scene, %d steps after set-up (%.1f s)"
-                      % (sc.n, steps, el)}
+    rec = {"value": sc.n * steps / el, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+           "sample": "FP64 oracle, rigid debris, same %d-particle scene, %d steps after set-up (%.1f s)"
+                     % (sc.n, steps, el)}
+    return rec, o, steps
+
+
+def _nrel(a, b):
+    d = np.linalg.norm(b)
+    return float(np.linalg.norm(a - b) / d) if d > 0 else float(np.linalg.norm(a - b))
+
+
+def balance(mass, vel, typ):
+    """Total mass and momentum of the moving particles (fluid and debris)."""
+    mv = (typ < 4)
+    return float(mass[mv].sum()), (mass[mv, None] * vel[mv]).sum(axis=0)
+
+
+def errors_vs_oracle(D, sc, params, prec, compensated, o, steps):
+    """Per-field errors of a fresh GPU run of `steps` steps against the FP64
+    oracle at the same step, plus mass and momentum balance of both."""
+    s = D.Solver(D.Params.from_dict(params), sc, precision=prec, rigid=True, compensated=compensated)
+    s.step(steps)
+    s.sync()
+    typ = o.field("type")
+    fl, so = typ < 2, (typ >= 2) & (typ < 4)
+    dx = s.field("pos") - o.field("pos")
+    L = sc.domain_max - sc.domain_min
+    dx -= L * np.round(dx / L)
+    n0p = o.scalars()["n0p"]
+    nd, ndr = s.field("vol_strain_p") + n0p, o.field("vol_strain_p") + n0p
+    m, mr = s.field("mass"), o.field("mass")
+    F, Fr = s.field("force"), o.field("force")
+    v, vr = s.field("vel"), o.field("vel")
+    b = s.bodies()
+    bx, bv = o.get("body_x").reshape(-1, 3), o.get("body_v").reshape(-1, 3)
+    M, P = balance(m, v, typ)
+    Mr, Pr = balance(mr, vr, typ)
+    out = {
+        "steps": steps,
+        "position_max_abs_m": float(np.abs(dx).max()),
+        "velocity_norm_rel": _nrel(v, vr),
+        "number_density_max_rel": float(np.max(np.abs(nd - ndr) / np.abs(ndr))),
+        "pressure_p_norm_rel": _nrel(s.field("pressure_p")[fl], o.field("pressure_p")[fl]),
+        "fluid_accel_norm_rel": _nrel(F[fl] / m[fl, None], Fr[fl] / mr[fl, None]),
+        "solid_force_norm_rel": _nrel(F[so], Fr[so]),
+        "body_position_max_abs_m": float(np.abs(b["x"] - bx).max()),
+        "body_velocity_norm_rel": _nrel(b["v"], bv),
+        "mass_total": M, "mass_total_oracle": Mr,
+        "momentum": [float(x) for x in P], "momentum_oracle": [float(x) for x in Pr],
+        "momentum_rel_diff": float(np.linalg.norm(P - Pr) / max(np.linalg.norm(Pr), 1e-300)),
+    }
+    s.close()
+    return out
 
 
 def copy_probe(mib: int) -> None:
@@ -210,7 +260,12 @@ def main(argv=None) -> int:
     }
     if not a.no_cpu_baseline:
         print("bench: timing the CPU baseline (bounded sample)", file=sys.stderr, flush=True)
-        out["cpu_baseline"] = cpu_baseline(sc, params, a.cpu_budget)
+        rec, o, k = cpu_baseline(sc, params, a.cpu_budget)
+        out["cpu_baseline"] = rec
+        print("bench: errors against the FP64 oracle after %d steps" % k, file=sys.stderr, flush=True)
+        out["errors_vs_fp64_oracle"] = {
+            v: errors_vs_oracle(D, sc, params, prec, v == "mixed", o, k) for v in variants}
+        o.close()
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out))

@@ -35,7 +35,7 @@ FIELDS = {
 }
 
 EXPORTED = ("dmph_default_options", "dmph_params_read", "dmph_grid_read", "dmph_grid_free", "dmph_create",
-            "dmph_step", "dmph_sync", "dmph_time", "dmph_get_field", "dmph_get_bodies", "dmph_get_info",
+            "dmph_step", "dmph_sync", "dmph_time", "dmph_get_field", "dmph_set_field", "dmph_get_bodies", "dmph_get_info",
             "dmph_virial", "dmph_pairs", "dmph_write_prof", "dmph_write_vtk", "dmph_set_timing", "dmph_get_timing",
             "dmph_last_error", "dmph_destroy")
 
@@ -252,6 +252,12 @@ class Solver:
         i = Info()
         self._check(lib().dmph_get_info(self.h, C.byref(i)), "dmph_get_info")
         return i
+
+    def set_field(self, name: str, values: np.ndarray, order: int = ORDER_ORIGINAL) -> None:
+        """Overwrite "pos" or "vel" (n x 3, float64) from host memory."""
+        fid, comps = FIELDS[name]
+        v = np.ascontiguousarray(values, np.float64).reshape(self.n, 3)
+        self._check(lib().dmph_set_field(self.h, fid, order, v.ctypes.data_as(C.c_void_p)), "dmph_set_field")
 
     def bodies(self) -> Dict[str, np.ndarray]:
         nb = self.info().nbodies

@@ -27,6 +27,7 @@ struct dmph_handle {
     virtual int sync() = 0;
     virtual int time(double *t) = 0;
     virtual int get_field(int field, int order, void *dst) = 0;
+    virtual int set_field(int field, int order, const void *src) = 0;
     virtual int get_bodies(double *x, double *v, double *q, double *l, double *m) = 0;
     virtual int info(dmph_info *out) = 0;
     virtual int virial() = 0;
@@ -718,6 +719,50 @@ class Engine final : public dmph_handle {
         return DMPH_OK;
     }
 
+    // Overwrite positions or velocities from host memory; positions are wrapped
+    // into the periodic domain and the next step rebins unconditionally.
+    int set_field(int field, int order, const void *src) override {
+        HCHK(hipSetDevice(device));
+        if (!src || (order != DMPH_ORDER_ORIGINAL && order != DMPH_ORDER_SLOT)) { err = "bad argument"; return DMPH_E_ARG; }
+        if (field != DMPH_FIELD_POSITION && field != DMPH_FIELD_VELOCITY) {
+            err = "only DMPH_FIELD_POSITION and DMPH_FIELD_VELOCITY can be set";
+            return DMPH_E_ARG;
+        }
+        const int n = g.n;
+        const double *x = (const double *)src;
+        std::vector<int> id, key;
+        TRY(download(id, a.id, n));
+        std::vector<R> c0(n), c1(n), c2(n);
+        if (field == DMPH_FIELD_POSITION && !std::is_same<R, double>::value) TRY(download(key, a.key, n));
+        for (int s = 0; s < n; s++) {
+            const double *v = x + 3 * (size_t)(order == DMPH_ORDER_ORIGINAL ? id[s] : s);
+            double w[3] = {v[0], v[1], v[2]};
+            if (field == DMPH_FIELD_POSITION) {
+                for (int k = 0; k < 3; k++) w[k] = modw(w[k] - g.dmin[k], g.dw[k]) + g.dmin[k];
+                if (!std::is_same<R, double>::value) {   // offset from the origin of the slot's current cell
+                    const int cell = key[s] % g.ncell;
+                    const int co[3] = {cell / (g.cc[1] * g.cc[2]), (cell / g.cc[2]) % g.cc[1], cell % g.cc[2]};
+                    for (int k = 0; k < 3; k++) {
+                        const double o_ = w[k] - g.dmin[k] - co[k] * g.cw[k];
+                        w[k] = modw(o_ + 0.5 * g.dw[k], g.dw[k]) - 0.5 * g.dw[k];
+                    }
+                }
+            }
+            c0[s] = (R)w[0]; c1[s] = (R)w[1]; c2[s] = (R)w[2];
+        }
+        R *d0 = field == DMPH_FIELD_POSITION ? a.px : a.vx, *d1 = field == DMPH_FIELD_POSITION ? a.py : a.vy,
+          *d2 = field == DMPH_FIELD_POSITION ? a.pz : a.vz;
+        HCHK(hipMemcpyAsync(d0, c0.data(), n * sizeof(R), hipMemcpyHostToDevice, stream));
+        HCHK(hipMemcpyAsync(d1, c1.data(), n * sizeof(R), hipMemcpyHostToDevice, stream));
+        HCHK(hipMemcpyAsync(d2, c2.data(), n * sizeof(R), hipMemcpyHostToDevice, stream));
+        if (field == DMPH_FIELD_POSITION) {
+            const int one = 1;
+            HCHK(hipMemcpyAsync(&a.sc->force, &one, sizeof one, hipMemcpyHostToDevice, stream));
+        }
+        HCHK(hipStreamSynchronize(stream));   // host vectors go out of scope
+        return DMPH_OK;
+    }
+
     int get_bodies(double *x, double *v, double *q, double *l, double *m) override {
         HCHK(hipSetDevice(device));
         const size_t nb = g.nb;
@@ -862,6 +907,9 @@ int dmph_create(const dmph_params *p, const dmph_options *opt, const dmph_grid *
 
 int dmph_step(dmph_handle *h, int nsteps) { return h && nsteps >= 0 ? h->step(nsteps) : DMPH_E_ARG; }
 int dmph_sync(dmph_handle *h) { return h ? h->sync() : DMPH_E_ARG; }
+int dmph_set_field(dmph_handle *h, int field, int order, const void *src) {
+    return h ? h->set_field(field, order, src) : DMPH_E_ARG;
+}
 int dmph_time(dmph_handle *h, double *t) { return h && t ? h->time(t) : DMPH_E_ARG; }
 int dmph_get_field(dmph_handle *h, int field, int order, void *dst) {
     return h ? h->get_field(field, order, dst) : DMPH_E_ARG;

@@ -44,6 +44,24 @@ def pmc_per_dispatch(db: str, counter: str) -> Dict[str, List[float]]:
     return out
 
 
+def pmc_means(db: str) -> Dict[str, Dict[str, float]]:
+    """Mean per dispatch of every counter in a --pmc database, per kernel, with
+    the mean dispatch duration of that run (ns) under "duration_ns"."""
+    c = sqlite3.connect(db)
+    per = collections.defaultdict(lambda: collections.defaultdict(float))
+    names = {}
+    for name, cn, v, did, du in c.execute(
+            "select name, counter_name, counter_value, dispatch_id, duration from pmc_events"):
+        per[did][cn] += v
+        per[did]["duration_ns"] = float(du)
+        names[did] = name
+    acc = collections.defaultdict(lambda: collections.defaultdict(list))
+    for did, cs in per.items():
+        for cn, v in cs.items():
+            acc[names[did]][cn].append(v)
+    return {k: {cn: sum(v) / len(v) for cn, v in cs.items()} for k, cs in acc.items()}
+
+
 def traffic(fetch_db: str, write_db: str, copy_mib: int) -> Tuple[Dict[str, Dict[str, float]], Dict[str, float]]:
     """Calibrated HBM bytes per dispatch per kernel, and the calibration factors."""
     f = pmc_per_dispatch(fetch_db, "FETCH_SIZE")
@@ -71,6 +89,9 @@ def main(argv=None) -> int:
     ap.add_argument("--out", required=True)
     ap.add_argument("--json")
     ap.add_argument("--top", type=int, default=14)
+    ap.add_argument("--counters", help="a --pmc database with SQ_ counters (VALU utilisation)")
+    ap.add_argument("--sclk-mhz", type=float, default=2400.0, help="shader clock for the utilisation estimate")
+    ap.add_argument("--simds", type=int, default=1024, help="SIMDs on the device (256 CUs x 4)")
     a = ap.parse_args(argv)
     st = kernel_stats(a.trace)
     lines = ["# " + a.title, ""]
@@ -92,6 +113,25 @@ def main(argv=None) -> int:
                                                           v["bytes"] / 1e6))
         js["traffic"] = tr
         js["calibration"] = fac
+    if a.counters:
+        pm = pmc_means(a.counters)
+        lines += ["", "Counters per dispatch (one `--pmc` pass).  VALU issue utilisation = SQ_INSTS_VALU x 4 cycles "
+                      "(a wave64 VALU instruction occupies a 16-lane SIMD for 4 cycles) / (%d SIMDs x the dispatch's "
+                      "cycles at %.0f MHz), durations from the same pass:" % (a.simds, a.sclk_mhz), "",
+                  "| kernel | waves | VALU instr / wave | VMEM reads / wave | LDS instr / wave | VALU issue util |",
+                  "|---|---|---|---|---|---|"]
+        js["counters"] = {}
+        for name, n, avg, tot in st[: a.top]:
+            c = pm.get(name)
+            if not c or "SQ_WAVES" not in c or c["SQ_WAVES"] <= 0:
+                continue
+            w = c["SQ_WAVES"]
+            cyc = c.get("duration_ns", 0.0) * a.sclk_mhz / 1e3
+            util = c.get("SQ_INSTS_VALU", 0.0) * 4.0 / (a.simds * cyc) if cyc > 0 else 0.0
+            lines.append("| `%s` | %d | %.0f | %.1f | %.1f | %.2f |" % (
+                name, w, c.get("SQ_INSTS_VALU", 0) / w, c.get("SQ_INSTS_VMEM_RD", 0) / w, c.get("SQ_INSTS_LDS", 0) / w,
+                util))
+            js["counters"][name] = dict(c, valu_issue_util=util)
     open(a.out, "w").write("\n".join(lines) + "\n")
     if a.json:
         json.dump(js, open(a.json, "w"), indent=1, sort_keys=True)

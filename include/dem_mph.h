@@ -139,6 +139,14 @@ enum {
 enum { DMPH_ORDER_ORIGINAL = 0, DMPH_ORDER_SLOT = 1 };
 int dmph_get_field(dmph_handle *h, int field, int order, void *dst);
 
+/* Overwrite DMPH_FIELD_POSITION or DMPH_FIELD_VELOCITY (double[n][3], in the
+ * given order) from host memory, e.g. to restart from a saved state (the
+ * reference re-reads a .prof as its .grid, source/DEM-MPH.cpp:681-851).
+ * Positions are wrapped into the periodic domain and the next step rebins
+ * unconditionally.  Members of rigid bodies are re-posed from their body at the
+ * next DEM sub-step.  Synchronous. */
+int dmph_set_field(dmph_handle *h, int field, int order, const void *src);
+
 /* Rigid-body state (no reference counterpart): per body X[3], V[3], q[4] (w,x,y,z), L[3], mass. */
 int dmph_get_bodies(dmph_handle *h, double *x, double *v, double *q, double *l, double *mass);
 
