@@ -186,6 +186,60 @@ def time_variant(D, sc, params, prec, compensated, steps, warmup):
     return {"seconds": el, "timing": tm, "info": info, "rebuilds": int(reb - reb0), "seconds_timed_launches": el_ev}
 
 
+def ensemble_bench(D, nscenes: int, steps: int, warmup: int, seed: int, sample: int):
+    """The batched ensemble: `nscenes` scenes in one handle (one graph per step
+    for all of them) against the plain FP32 solver advancing the same scenes one
+    handle at a time (timed on `sample` evenly spaced scenes, scaled to all)."""
+    from dem_mph_amd import scenes as S
+    t0 = time.perf_counter()
+    scenes = S.ensemble_c3(nscenes, seed=seed)
+    t_gen = time.perf_counter() - t0
+    params = S.params_c3()
+    dt = params["Dt"]
+    n = sum(sc.n for sc in scenes)
+    out = {"workload": "ensemble_c3: %d dam-break scenes in the C1 tank at d = 0.01 m, fluid 2k-60k (log-uniform), "
+                       "1-32 rigid cuboids of side 3-8, dt = 5e-5 s" % nscenes,
+           "scenes": nscenes, "particles": n, "steps": steps, "generation_s": t_gen}
+    res = {}
+    for name, comp in (("batched_mixed", True), ("batched_plain", False)):
+        t0 = time.perf_counter()
+        s = D.Solver(D.Params.from_dict(params), scenes, precision=D.PREC_FP32, rigid=True, compensated=comp)
+        t_create = time.perf_counter() - t0
+        s.step(warmup)
+        s.sync()
+        t0 = time.perf_counter()
+        s.step(steps)
+        s.sync()
+        el = time.perf_counter() - t0
+        s.close()
+        res[name] = {"particle_steps_per_s": n * steps / el, "ms_per_step": el / steps * 1e3,
+                     "sim_s_per_wall_s": dt * steps / el, "create_s": t_create}
+        print("bench: ensemble %s %.2f ms/step" % (name, el / steps * 1e3), file=sys.stderr, flush=True)
+    # plain FP32 solver, one scene per handle, scenes one after another
+    idx = np.linspace(0, nscenes - 1, min(sample, nscenes)).round().astype(int)
+    el = 0.0
+    n_s = 0
+    for k in idx:
+        s = D.Solver(D.Params.from_dict(params), scenes[k], precision=D.PREC_FP32, rigid=True, compensated=False)
+        s.step(warmup)
+        s.sync()
+        t0 = time.perf_counter()
+        s.step(steps)
+        s.sync()
+        el += time.perf_counter() - t0
+        n_s += scenes[k].n
+        s.close()
+    scale = n / n_s   # sample time scaled by particle count to the whole ensemble
+    el_all = el * scale
+    res["sequential_plain"] = {"particle_steps_per_s": n * steps / el_all, "ms_per_step": el_all / steps * 1e3,
+                               "sim_s_per_wall_s": dt * steps / el_all,
+                               "sample": "%d of %d scenes (%d particles), time scaled by particle count"
+                                         % (len(idx), nscenes, n_s)}
+    out.update(res)
+    out["batched_over_sequential"] = res["batched_mixed"]["sim_s_per_wall_s"] / res["sequential_plain"]["sim_s_per_wall_s"]
+    return out
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -196,6 +250,10 @@ def main(argv=None) -> int:
     ap.add_argument("--no-plain", action="store_true", help="skip the plain variant (FP32 sums)")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble measurement")
+    ap.add_argument("--ensemble-scenes", type=int, default=512)
+    ap.add_argument("--ensemble-steps", type=int, default=40)
+    ap.add_argument("--ensemble-sample", type=int, default=32)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="also run one device-to-device copy of MIB MiB (counter calibration)")
     a = ap.parse_args(argv)
@@ -268,6 +326,9 @@ def main(argv=None) -> int:
         o.close()
     else:
         out["cpu_baseline"] = None
+    if not a.no_ensemble:
+        print("bench: batched ensemble", file=sys.stderr, flush=True)
+        out["ensemble"] = ensemble_bench(D, a.ensemble_scenes, a.ensemble_steps, 5, a.seed, a.ensemble_sample)
     print(json.dumps(out))
     return 0
 

@@ -35,7 +35,8 @@ FIELDS = {
 }
 
 EXPORTED = ("dmph_default_options", "dmph_params_read", "dmph_grid_read", "dmph_grid_free", "dmph_create",
-            "dmph_step", "dmph_sync", "dmph_time", "dmph_get_field", "dmph_set_field", "dmph_get_bodies", "dmph_get_info",
+            "dmph_create_batch", "dmph_step", "dmph_sync", "dmph_time", "dmph_get_field", "dmph_set_field",
+            "dmph_get_bodies", "dmph_get_info",
             "dmph_virial", "dmph_pairs", "dmph_write_prof", "dmph_write_vtk", "dmph_set_timing", "dmph_get_timing",
             "dmph_last_error", "dmph_destroy")
 
@@ -115,7 +116,8 @@ class Info(C.Structure):
                 ("nbodies", C.c_int), ("substeps", C.c_int), ("precision", C.c_int),
                 ("fluid_begin", C.c_int), ("fluid_end", C.c_int), ("solid_begin", C.c_int), ("solid_end", C.c_int),
                 ("wall_begin", C.c_int), ("wall_end", C.c_int), ("cell_width", C.c_double * 3),
-                ("n0a", C.c_double), ("n0p", C.c_double), ("max_radius", C.c_double), ("rebuilds", C.c_int64)]
+                ("n0a", C.c_double), ("n0p", C.c_double), ("max_radius", C.c_double), ("rebuilds", C.c_int64),
+                ("nscenes", C.c_int)]
 
 
 _lib = None
@@ -186,16 +188,40 @@ def read_grid(path: str):
 
 
 class Solver:
-    """One DEM-MPH scene on one GPU, driven through the C ABI."""
+    """One DEM-MPH scene on one GPU, driven through the C ABI -- or, given a
+    list of scenes that share spacing and domain, a batch advanced together
+    (dmph_create_batch); fields then cover all scenes, scene after scene
+    (`scene_offsets`)."""
 
     def __init__(self, params: Params, scene, precision: int = PREC_FP64, sort_mode: int = SORT_CANONICAL,
                  rebuild: int = REBUILD_SKIN, rigid: bool = False, dim: Optional[int] = None, device: int = 0,
                  compensated: bool = True, graph: bool = True):
         self._arrays = []
+        scenes = list(scene) if isinstance(scene, (list, tuple)) else [scene]
+        grids = (Grid * len(scenes))()
+        for k, sc in enumerate(scenes):
+            self._fill_grid(grids[k], sc)
+        first = scenes[0]
+        self.options = default_options(dim=dim if dim else (first.dim if first.dim in (2, 3) else 0),
+                                       precision=precision, sort_mode=sort_mode, rebuild=rebuild,
+                                       debris=DEBRIS_RIGID if rigid else DEBRIS_PARTICLES, device=device,
+                                       compensated=1 if compensated else 0, graph=1 if graph else 0)
+        self.params = params
+        h = C.c_void_p()
+        if len(scenes) == 1:
+            rc = lib().dmph_create(C.byref(params), C.byref(self.options), C.byref(grids[0]), C.byref(h))
+        else:
+            rc = lib().dmph_create_batch(C.byref(params), C.byref(self.options), grids, len(scenes), C.byref(h))
+        if rc != OK:
+            raise DmphError("dmph_create failed (%d): %s" % (rc, lib().dmph_last_error(None).decode()))
+        self.h = h
+        self.n = sum(sc.n for sc in scenes)
+        self.scene_offsets = np.cumsum([0] + [sc.n for sc in scenes])
+
+    def _fill_grid(self, g: "Grid", scene) -> None:
         t = np.ascontiguousarray(scene.type, np.int32)
         pos = np.ascontiguousarray(scene.pos, np.float64)
         vel = np.ascontiguousarray(scene.vel, np.float64)
-        g = Grid()
         g.time, g.n, g.spacing = scene.time, scene.n, scene.d
         for a in range(3):
             g.domain_min[a] = scene.domain_min[a]
@@ -212,17 +238,6 @@ class Solver:
             dn = np.ascontiguousarray(scene.density, np.float64)
             g.density = dn.ctypes.data_as(C.POINTER(C.c_double))
             self._arrays.append(dn)
-        self.options = default_options(dim=dim if dim else (scene.dim if scene.dim in (2, 3) else 0),
-                                       precision=precision, sort_mode=sort_mode, rebuild=rebuild,
-                                       debris=DEBRIS_RIGID if rigid else DEBRIS_PARTICLES, device=device,
-                                       compensated=1 if compensated else 0, graph=1 if graph else 0)
-        self.params = params
-        h = C.c_void_p()
-        rc = lib().dmph_create(C.byref(params), C.byref(self.options), C.byref(g), C.byref(h))
-        if rc != OK:
-            raise DmphError("dmph_create failed (%d): %s" % (rc, lib().dmph_last_error(None).decode()))
-        self.h = h
-        self.n = scene.n
 
     def _check(self, rc: int, what: str) -> None:
         if rc != OK:

@@ -2,7 +2,10 @@
 //
 // Particles live in "slots": structure-of-arrays sorted by the reference's cell
 // key (cell id + block * cells, block 0 fluid / 1 solid / 2 wall;
-// source/DEM-MPH.cpp:1292-1313).  Block ranges never change after the first
+// source/DEM-MPH.cpp:1292-1313).  A batch of scenes that share one domain adds
+// the scene as a middle digit: key = block * T + scene * cells + cell with
+// T = scenes * cells, so one set of launches advances every scene and a
+// stencil never leaves its scene.  Block ranges never change after the first
 // binning (the type counts are fixed), so [fb,fe) [sb,se) [wb,we) are host
 // constants.  Per-particle state is permuted at every rebinning; the derived
 // per-slot arrays (pressures, DEM coefficients, ...) stay where they are, as
@@ -25,6 +28,8 @@ constexpr int kNType = 6;
 // Uniform parameters of one scene (passed by value to every kernel).
 struct Geo {
     int n, ncell, dim, nb, substeps;
+    int nscene;         // independent scenes sharing the domain (a batch); 1 for a single scene
+    int T;              // cells of one block over all scenes: nscene * ncell
     int cc[3];
     int fb, fe, sb, se, wb, we;
     int st_active;      // any non-zero InteractionRatio: surface-tension passes contribute
@@ -75,9 +80,9 @@ struct Dev {
     int *nnear;                    // FP32: entries in the near segment of each row
     uint32_t *clist;               // [ccap * (n - sb)] DEM contact entries of solid and wall slots
     int *ncontact;                 // [n - sb]
-    int *cell_start;               // [3*ncell + 1]
+    int *cell_start;               // [3*T + 1]
     // rebinning scratch
-    int *counts, *fill;            // [3*ncell + 1]
+    int *counts, *fill;            // [3*T + 1]
     int *key_new, *order, *perm;   // [n]
     int *bkey, *bval;              // [p2] bitonic
     int *block_sums;               // scan carries

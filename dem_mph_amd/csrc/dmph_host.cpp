@@ -178,11 +178,12 @@ static int domain(Geo &g, const dmph_grid &gr, std::string &err) {
         return DMPH_E_DOMAIN;
     }
     const double cells = cnt[0] * cnt[1] * cnt[2];
-    if (cells * 3 > 2.0e9) {
+    if (cells * 3 * g.nscene > 2.0e9) {
         err = "too many cells";
         return DMPH_E_DOMAIN;
     }
     g.ncell = (int)cells;
+    g.T = g.ncell * g.nscene;
     for (int a = 0; a < 3; a++) g.cw[a] = g.dw[a] / g.cc[a];
     for (int a = 0; a < g.dim; a++)
         if ((int)ceil((g.rmax + margin) / g.cw[a]) != 1) {
@@ -238,9 +239,12 @@ class Engine final : public dmph_handle {
 
 #define TRY(x) do { int rc_ = (x); if (rc_ != DMPH_OK) return rc_; } while (0)
 
-    int create(const dmph_params *pp, const dmph_options *op, const dmph_grid *gr) {
+    // scene_of: scene of each particle of a batch (nscene > 1), or null
+    int create(const dmph_params *pp, const dmph_options *op, const dmph_grid *gr, const int *scene_of = nullptr,
+               int nscene = 1) {
         p = *pp;
         o = *op;
+        g.nscene = nscene;
         const int n = gr->n;
         if (n <= 0 || !gr->type || !gr->pos || !gr->vel || gr->spacing <= 0.0) {
             err = "empty or invalid particle set";
@@ -314,7 +318,7 @@ class Engine final : public dmph_handle {
         }
         g.rigid = o.debris == DMPH_DEBRIS_RIGID && gr->body != nullptr;
         TRY(allocate());
-        TRY(upload(*gr));
+        TRY(upload(*gr, scene_of));
         // first binning + the pre-loop density passes (source/DEM-MPH.cpp:419-425)
         Scalars sc{};
         sc.time = gr->time;
@@ -331,7 +335,7 @@ class Engine final : public dmph_handle {
     }
 
     int allocate() {
-        const size_t n = g.n, nk = 3 * (size_t)g.ncell + 1;
+        const size_t n = g.n, nk = 3 * (size_t)g.T + 1;
         TRY(alloc(a.id, n)); TRY(alloc(a.type, n)); TRY(alloc(a.body, n)); TRY(alloc(a.key, n));
         R **reals[] = {&a.px, &a.py, &a.pz, &a.vx, &a.vy, &a.vz, &a.rho, &a.fx, &a.fy, &a.fz, &a.bx, &a.by, &a.bz,
                        &a.dens_a, &a.gcx, &a.gcy, &a.gcz, &a.vs_p, &a.div_p, &a.press_p, &a.press_a, &a.mass,
@@ -355,9 +359,12 @@ class Engine final : public dmph_handle {
     }
 
     // host copies in file order -> slots (slot = file index until the first binning)
-    int upload(const dmph_grid &gr) {
+    int upload(const dmph_grid &gr, const int *scene_of) {
         const int n = g.n;
         std::vector<int> id(n), type(n), body(n, -1), key(n, 0);
+        // the scene rides in the key (scene * cells + cell) from the start: k_keys reads it back
+        if (scene_of)
+            for (int i = 0; i < n; i++) key[i] = scene_of[i] * g.ncell;
         std::vector<R> px(n), py(n), pz(n), vx(n), vy(n), vz(n), rho(n), mass(n), kappa(n);
         std::vector<double> rhod(n);
         for (int i = 0; i < n; i++) {
@@ -381,7 +388,7 @@ class Engine final : public dmph_handle {
                     const double o_ = x[k] - g.dmin[k] - c[k] * g.cw[k];
                     off[k] = modw(o_ + 0.5 * g.dw[k], g.dw[k]) - 0.5 * g.dw[k];
                 }
-                key[i] = (c[0] * g.cc[1] + c[1]) * g.cc[2] + c[2];
+                key[i] += (c[0] * g.cc[1] + c[1]) * g.cc[2] + c[2];
                 px[i] = (R)off[0]; py[i] = (R)off[1]; pz[i] = (R)off[2];
             }
         }
@@ -784,6 +791,7 @@ class Engine final : public dmph_handle {
         out->fluid_begin = g.fb; out->fluid_end = g.fe; out->solid_begin = g.sb; out->solid_end = g.se;
         out->wall_begin = g.wb; out->wall_end = g.we;
         out->n0a = g.n0a; out->n0p = g.n0p; out->max_radius = g.rmax;
+        out->nscenes = g.nscene;
         Scalars sc;
         HCHK(hipMemcpyAsync(&sc, a.sc, sizeof sc, hipMemcpyDeviceToHost, stream));
         HCHK(hipStreamSynchronize(stream));
@@ -892,6 +900,85 @@ int dmph_create(const dmph_params *p, const dmph_options *opt, const dmph_grid *
     } else if (o.precision == DMPH_PREC_FP64) {
         auto e = std::make_unique<dmph::Engine<double>>();
         rc = e->create(p, &o, g);
+        h = std::move(e);
+    } else {
+        g_create_error = "unknown precision";
+        return DMPH_E_ARG;
+    }
+    if (rc != DMPH_OK) {
+        g_create_error = h->err;
+        return rc;
+    }
+    *out = h.release();
+    return DMPH_OK;
+}
+
+int dmph_create_batch(const dmph_params *p, const dmph_options *opt, const dmph_grid *grids, int nscenes,
+                      dmph_handle **out) {
+    if (!p || !grids || nscenes <= 0 || !out) return DMPH_E_ARG;
+    *out = nullptr;
+    if (nscenes == 1) return dmph_create(p, opt, grids, out);
+    const dmph_grid &g0 = grids[0];
+    int64_t n = 0;
+    bool bodies = false, dens = false;
+    for (int s = 0; s < nscenes; s++) {
+        const dmph_grid &gs = grids[s];
+        if (gs.n <= 0 || !gs.type || !gs.pos || !gs.vel) { g_create_error = "empty or invalid scene"; return DMPH_E_ARG; }
+        if (gs.spacing != g0.spacing || gs.time != g0.time) {
+            g_create_error = "scenes of a batch need the same spacing and time";
+            return DMPH_E_ARG;
+        }
+        for (int k = 0; k < 3; k++)
+            if (gs.domain_min[k] != g0.domain_min[k] || gs.domain_max[k] != g0.domain_max[k]) {
+                g_create_error = "scenes of a batch need the same domain";
+                return DMPH_E_ARG;
+            }
+        n += gs.n;
+        bodies = bodies || gs.body;
+        dens = dens || gs.density;
+    }
+    if (n >= (1 << 26)) { g_create_error = "batch larger than 2^26 particles"; return DMPH_E_ARG; }
+    // one combined particle set, scene after scene; body ids offset per scene
+    std::vector<int> type(n), body(bodies ? n : 0), scene(n);
+    std::vector<double> pos(3 * n), vel(3 * n), rho(dens ? n : 0);
+    int64_t off = 0;
+    int boff = 0;
+    for (int s = 0; s < nscenes; s++) {
+        const dmph_grid &gs = grids[s];
+        int nb = 0;
+        for (int i = 0; i < gs.n; i++) {
+            type[off + i] = gs.type[i];
+            scene[off + i] = s;
+            for (int k = 0; k < 3; k++) { pos[3 * (off + i) + k] = gs.pos[3 * i + k]; vel[3 * (off + i) + k] = gs.vel[3 * i + k]; }
+            if (bodies) {
+                const int b = gs.body ? gs.body[i] : -1;
+                body[off + i] = b >= 0 ? b + boff : -1;
+                nb = b + 1 > nb ? b + 1 : nb;
+            }
+            if (dens) rho[off + i] = gs.density ? gs.density[i] : -1.0;
+        }
+        off += gs.n;
+        boff += nb;
+    }
+    dmph_grid all = g0;
+    all.n = (int)n;
+    all.type = type.data();
+    all.pos = pos.data();
+    all.vel = vel.data();
+    all.body = bodies ? body.data() : nullptr;
+    all.density = dens ? rho.data() : nullptr;
+    dmph_options o;
+    if (opt) o = *opt;
+    else dmph_default_options(&o);
+    std::unique_ptr<dmph_handle> h;
+    int rc;
+    if (o.precision == DMPH_PREC_FP32) {
+        auto e = std::make_unique<dmph::Engine<float>>();
+        rc = e->create(p, &o, &all, scene.data(), nscenes);
+        h = std::move(e);
+    } else if (o.precision == DMPH_PREC_FP64) {
+        auto e = std::make_unique<dmph::Engine<double>>();
+        rc = e->create(p, &o, &all, scene.data(), nscenes);
         h = std::move(e);
     } else {
         g_create_error = "unknown precision";

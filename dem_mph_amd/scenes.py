@@ -324,6 +324,59 @@ def dam_break_c2(seed: int = 0, rigid: bool = True) -> Scene:
     return dam_break(d, interior, fluid, layers=3, debris=deb, dim=3, seed=seed, rigid=rigid)
 
 
+def ensemble_c3(nscenes: int = 512, seed: int = 0, d: float = 0.01,
+                interior: Sequence[float] = (0.8, 0.4, 0.6),
+                fluid_range: Tuple[float, float] = (2000.0, 60000.0)) -> List[Scene]:
+    """Batched ensemble: ``nscenes`` independent dam breaks in one shared tank.
+
+    Every scene has the C1 tank (0.8 x 0.4 x 0.6 m interior, three wall
+    layers, open top) at d = 0.01 m and the same periodic domain -- taken from
+    the unjittered walls, so the scenes can be batched -- but its own:
+      - fluid count N log-uniform in ``fluid_range`` ([2,000, 60,000]): a column spanning the tank
+        depth (40 particles) whose height / width ratio is uniform in [0.5, 2];
+      - 1-32 rigid cuboids with sides uniform in {3..8} particles and densities
+        uniform in [400, 950] kg/m^3, in slots of a 9 d grid on the floor
+        downstream of the column (at least 3 d from the fluid, beyond RadiusP),
+        first layer first, then stacked layers;
+      - jitter seed.
+    """
+    L = np.asarray(interior, float)
+    nx, ny, nz = (int(round(x / d)) for x in L)
+    walls = tank_walls(L, d, 3, False, 3)
+    dmin, dmax = _pad_domain(walls.min(axis=0), walls.max(axis=0), d, 3.0, 3)
+    out = []
+    for s in range(nscenes):
+        u = uniform(seed, 4, stream=1000 + 4 * s)
+        lo_n, hi_n = fluid_range
+        nf = float(np.exp(np.log(lo_n) + u[0] * (np.log(hi_n) - np.log(lo_n))))
+        aspect = 0.5 + 1.5 * u[1]
+        fy = ny
+        fx = int(np.clip(round(np.sqrt(nf / (fy * aspect))), 4, nx // 2))
+        fz = int(np.clip(round(nf / (fy * fx)), 4, nz - 6))
+        ncub = 1 + min(int(u[2] * 32), 31)
+        x0 = fx + 3
+        cols = max((nx - x0 - 8) // 9 + 1, 0)   # slots whose 8-wide cuboid fits inside the tank
+        rows = max((ny - 8) // 9 + 1, 0)
+        slots = [(x0 + 9 * c, 9 * r, 9 * z) for z in range(3) for c in range(cols) for r in range(rows)]
+        per_layer = cols * rows
+        order = []
+        for z in range(3):   # seeded order inside each layer, lower layers first
+            perm = np.argsort(uniform(seed, per_layer, stream=1001 + 4 * s + z * 7919))
+            order += [z * per_layer + int(k) for k in perm]
+        ncub = min(ncub, len(order))   # small tanks hold fewer slots than cuboids
+        sides = 3 + np.minimum((uniform(seed, 3 * ncub, stream=1002 + 4 * s) * 6).astype(int), 5)
+        rho = uniform(seed, ncub, stream=1003 + 4 * s, lo=400.0, hi=950.0)
+        deb = []
+        for k in range(ncub):
+            sx, sy, sz = slots[order[k]]
+            deb.append(((sx * d, sy * d, sz * d), tuple(int(v) for v in sides[3 * k: 3 * k + 3]), float(rho[k])))
+        sc = dam_break(d, L, (fx * d, fy * d, fz * d), layers=3, debris=deb, dim=3,
+                       seed=seed * 1000003 + s, rigid=True)
+        sc.domain_min, sc.domain_max = dmin.copy(), dmax.copy()
+        out.append(sc)
+    return out
+
+
 def small_2d(seed: int = 0, debris_velocity=(0.3, -0.2, 0.0), fluid_velocity=(0.0, 0.0, 0.0),
              nfx: int = 16, nfy: int = 12, d: float = 0.002, gate: bool = True) -> Scene:
     """Small 2D tank (one particle layer in z), shaped like the reference's shipped case.
@@ -356,6 +409,13 @@ def params_c1(**over) -> Dict[str, object]:
                     OutputInterval=0.1, VtkOutputInterval=0.1)
     p["Density"][2] = 800.0
     p["YoungModulus"][2] = 2.0e7
+    p.update(over)
+    return p
+
+
+def params_c3(**over) -> Dict[str, object]:
+    """C3 parameters: as C1 at d = 0.01 m, dt = DEMDt = 5e-5 s (the same acoustic CFL as C1 and C2)."""
+    p = params_c1(Dt=5.0e-5, DEMDt=5.0e-5, EndTime=0.15, OutputInterval=1.0, VtkOutputInterval=1.0)
     p.update(over)
     return p
 

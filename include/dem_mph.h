@@ -94,6 +94,18 @@ typedef struct dmph_handle dmph_handle;
  * DensityP passes. */
 int dmph_create(const dmph_params *p, const dmph_options *opt, const dmph_grid *g, dmph_handle **out);
 
+/* A batch of independent scenes advanced together (no reference counterpart:
+ * the reference runs one scene per process).  All grids must have the same
+ * spacing and domain; particle counts, fluid columns and bodies may differ.
+ * Scene s occupies the original-order indices [sum of n of scenes < s, + n_s)
+ * of every field; rigid-body ids are per scene (body b of scene s is body
+ * (bodies of scenes < s) + b in dmph_get_bodies).  The handle steps every scene
+ * with one set of launches; a scene's particles never interact with another's.
+ * With canonical order and rebinning every step, each scene evolves bit for bit
+ * as it would in its own handle. */
+int dmph_create_batch(const dmph_params *p, const dmph_options *opt, const dmph_grid *grids, int nscenes,
+                      dmph_handle **out);
+
 /* `nsteps` passes of the main loop body (source/DEM-MPH.cpp:450-579): fluid
  * convection, walls, periodic wrap, rebuild on the 0.05 d skin test, the MPH
  * passes, drag (twice), buoyancy, gravity, acceleration and the DEM sub-steps.
@@ -155,6 +167,7 @@ typedef struct dmph_info {
     int fluid_begin, fluid_end, solid_begin, solid_end, wall_begin, wall_end;
     double cell_width[3], n0a, n0p, max_radius;
     int64_t rebuilds;   /* binning passes so far (device counter) */
+    int nscenes;        /* scenes in the handle (dmph_create_batch), 1 otherwise; cells is per scene */
 } dmph_info;
 int dmph_get_info(dmph_handle *h, dmph_info *info);
 

@@ -1,0 +1,64 @@
+"""Scene generators (CPU): the workloads' stated shapes, counts and separations."""
+import numpy as np
+import pytest
+
+from dem_mph_amd import scenes as S
+
+
+def test_c2_counts():
+    sc = S.dam_break_c2(seed=1)
+    c = sc.counts()
+    assert (c["fluid"], c["solid"], c["wall"], c["bodies"]) == (562500, 64000, 340908, 64)
+    rho = sc.density[sc.type == S.SOLID]
+    assert 500.0 <= rho.min() and rho.max() <= 900.0
+
+
+@pytest.fixture(scope="module")
+def ensemble():
+    return S.ensemble_c3(512, seed=0)
+
+
+def test_ensemble_shares_one_domain_and_spacing(ensemble):
+    lo, hi = ensemble[0].domain_min, ensemble[0].domain_max
+    for sc in ensemble:
+        assert sc.d == 0.01
+        np.testing.assert_array_equal(sc.domain_min, lo)
+        np.testing.assert_array_equal(sc.domain_max, hi)
+        assert (sc.pos > lo).all() and (sc.pos < hi).all()
+
+
+def test_ensemble_fluid_counts_log_uniform(ensemble):
+    nf = np.array([(sc.type < 2).sum() for sc in ensemble])
+    # log-uniform on [2000, 60000]: mean (60000 - 2000) / ln 30 = 17,053; column rounding shifts it slightly
+    assert 1900 <= nf.min() and nf.max() <= 61000
+    assert 15000 < nf.mean() < 19000
+    # roughly uniform in log N: each of three log-equal bins holds about a third
+    b = np.digitize(np.log(nf), np.log([2000.0, 6214.0, 19311.0, 60001.0]))
+    frac = np.bincount(b, minlength=5)[1:4] / len(nf)
+    assert (frac > 0.25).all()
+
+
+def test_ensemble_debris_bodies(ensemble):
+    nb = np.array([sc.body.max() + 1 for sc in ensemble])
+    assert nb.min() >= 1 and nb.max() <= 32
+    for sc in ensemble[:64]:
+        sol = (sc.type >= 2) & (sc.type < 4)
+        body = sc.body[sol]
+        for b in np.unique(body):
+            pts = sc.pos[sol][body == b]
+            ext = np.round((pts.max(axis=0) - pts.min(axis=0)) / sc.d).astype(int) + 1
+            assert ((3 <= ext) & (ext <= 8)).all()
+            r = np.unique(sc.density[sol][body == b])
+            assert len(r) == 1 and 400.0 <= r[0] <= 950.0
+        # debris starts beyond RadiusP (2.5 d) of the fluid column (the drag is NaN at zero relative velocity)
+        fl = sc.pos[sc.type < 2]
+        assert sc.pos[sol][:, 0].min() - fl[:, 0].max() > 2.5 * sc.d
+
+
+def test_ensemble_is_seeded():
+    a = S.ensemble_c3(3, seed=5, d=0.02, interior=(0.6, 0.3, 0.4), fluid_range=(200.0, 3000.0))
+    b = S.ensemble_c3(3, seed=5, d=0.02, interior=(0.6, 0.3, 0.4), fluid_range=(200.0, 3000.0))
+    c = S.ensemble_c3(3, seed=6, d=0.02, interior=(0.6, 0.3, 0.4), fluid_range=(200.0, 3000.0))
+    for x, y in zip(a, b):
+        np.testing.assert_array_equal(x.pos, y.pos)
+    assert any(x.n != z.n or not np.array_equal(x.pos, z.pos) for x, z in zip(a, c))
