@@ -169,6 +169,9 @@ def time_variant(D, sc, params, prec, compensated, steps, warmup):
     info = s.info()
     s.step(warmup)
     s.sync()
+    lanes = s.autotune(5)   # lanes per particle of the two sweeps, timed on this scene
+    s.step(warmup)
+    s.sync()
     reb0 = s.info().rebuilds
     t0 = time.perf_counter()
     s.step(steps)
@@ -183,7 +186,8 @@ def time_variant(D, sc, params, prec, compensated, steps, warmup):
     tm = s.timing()
     s.set_timing(False)
     s.close()
-    return {"seconds": el, "timing": tm, "info": info, "rebuilds": int(reb - reb0), "seconds_timed_launches": el_ev}
+    return {"seconds": el, "timing": tm, "info": info, "rebuilds": int(reb - reb0), "seconds_timed_launches": el_ev,
+            "lanes": list(lanes)}
 
 
 def ensemble_bench(D, nscenes: int, steps: int, warmup: int, seed: int, sample: int):
@@ -207,13 +211,16 @@ def ensemble_bench(D, nscenes: int, steps: int, warmup: int, seed: int, sample: 
         t_create = time.perf_counter() - t0
         s.step(warmup)
         s.sync()
+        lanes = s.autotune(3)
+        s.step(2)
+        s.sync()
         t0 = time.perf_counter()
         s.step(steps)
         s.sync()
         el = time.perf_counter() - t0
         s.close()
         res[name] = {"particle_steps_per_s": n * steps / el, "ms_per_step": el / steps * 1e3,
-                     "sim_s_per_wall_s": dt * steps / el, "create_s": t_create}
+                     "sim_s_per_wall_s": dt * steps / el, "create_s": t_create, "autotuned_lanes": list(lanes)}
         print("bench: ensemble %s %.2f ms/step" % (name, el / steps * 1e3), file=sys.stderr, flush=True)
     # plain FP32 solver, one scene per handle, scenes one after another
     idx = np.linspace(0, nscenes - 1, min(sample, nscenes)).round().astype(int)
@@ -310,7 +317,8 @@ def main(argv=None) -> int:
         "kernel_ms_per_step": {k: v[0] / a.steps for k, v in groups.items()},
         "ms_per_step_timed_launches": mixed["seconds_timed_launches"] / a.steps * 1e3,
         "variants": {k: {"particle_steps_per_s": n * a.steps / v["seconds"], "ms_per_step": v["seconds"] / a.steps * 1e3,
-                         "sim_s_per_wall_s": dt * a.steps / v["seconds"]} for k, v in variants.items()},
+                         "sim_s_per_wall_s": dt * a.steps / v["seconds"], "autotuned_lanes": v["lanes"]}
+                     for k, v in variants.items()},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": kernel_traffic(kname),
                      "algorithmic_bytes_per_unit": ALGO_BYTES[dom], "units_per_launch": units,

@@ -38,6 +38,7 @@ EXPORTED = ("dmph_default_options", "dmph_params_read", "dmph_grid_read", "dmph_
             "dmph_create_batch", "dmph_step", "dmph_sync", "dmph_time", "dmph_get_field", "dmph_set_field",
             "dmph_get_bodies", "dmph_get_info",
             "dmph_virial", "dmph_pairs", "dmph_write_prof", "dmph_write_vtk", "dmph_set_timing", "dmph_get_timing",
+            "dmph_autotune",
             "dmph_last_error", "dmph_destroy")
 
 TIMING_GROUPS = ("predict", "rebin", "sweep1", "sweep2", "accel", "dem", "move", "end")
@@ -108,7 +109,8 @@ class Grid(C.Structure):
 
 class Options(C.Structure):
     _fields_ = [("dim", C.c_int), ("precision", C.c_int), ("sort_mode", C.c_int), ("rebuild", C.c_int),
-                ("debris", C.c_int), ("device", C.c_int), ("compensated", C.c_int), ("graph", C.c_int)]
+                ("debris", C.c_int), ("device", C.c_int), ("compensated", C.c_int), ("graph", C.c_int),
+                ("lanes", C.c_int)]
 
 
 class Info(C.Structure):
@@ -117,7 +119,7 @@ class Info(C.Structure):
                 ("fluid_begin", C.c_int), ("fluid_end", C.c_int), ("solid_begin", C.c_int), ("solid_end", C.c_int),
                 ("wall_begin", C.c_int), ("wall_end", C.c_int), ("cell_width", C.c_double * 3),
                 ("n0a", C.c_double), ("n0p", C.c_double), ("max_radius", C.c_double), ("rebuilds", C.c_int64),
-                ("nscenes", C.c_int)]
+                ("nscenes", C.c_int), ("lanes", C.c_int * 2)]
 
 
 _lib = None
@@ -195,7 +197,7 @@ class Solver:
 
     def __init__(self, params: Params, scene, precision: int = PREC_FP64, sort_mode: int = SORT_CANONICAL,
                  rebuild: int = REBUILD_SKIN, rigid: bool = False, dim: Optional[int] = None, device: int = 0,
-                 compensated: bool = True, graph: bool = True):
+                 compensated: bool = True, graph: bool = True, lanes: int = 0):
         self._arrays = []
         scenes = list(scene) if isinstance(scene, (list, tuple)) else [scene]
         grids = (Grid * len(scenes))()
@@ -205,7 +207,7 @@ class Solver:
         self.options = default_options(dim=dim if dim else (first.dim if first.dim in (2, 3) else 0),
                                        precision=precision, sort_mode=sort_mode, rebuild=rebuild,
                                        debris=DEBRIS_RIGID if rigid else DEBRIS_PARTICLES, device=device,
-                                       compensated=1 if compensated else 0, graph=1 if graph else 0)
+                                       compensated=1 if compensated else 0, graph=1 if graph else 0, lanes=lanes)
         self.params = params
         h = C.c_void_p()
         if len(scenes) == 1:
@@ -267,6 +269,12 @@ class Solver:
         i = Info()
         self._check(lib().dmph_get_info(self.h, C.byref(i)), "dmph_get_info")
         return i
+
+    def autotune(self, reps: int = 5) -> tuple:
+        """Pick lanes per particle for the FP32 sweeps by timing; returns (sweep1, sweep2)."""
+        lanes = (C.c_int * 2)()
+        self._check(lib().dmph_autotune(self.h, reps, lanes), "dmph_autotune")
+        return lanes[0], lanes[1]
 
     def set_field(self, name: str, values: np.ndarray, order: int = ORDER_ORIGINAL) -> None:
         """Overwrite "pos" or "vel" (n x 3, float64) from host memory."""

@@ -35,6 +35,7 @@ struct dmph_handle {
     virtual int write_prof(const char *path) = 0;
     virtual int write_vtk(const char *path) = 0;
     virtual int set_timing(int on) = 0;
+    virtual int autotune(int reps, int *lanes) = 0;
     virtual int get_timing(int group, double *ms, int64_t *launches) = 0;
 };
 
@@ -273,6 +274,7 @@ class Engine final : public dmph_handle {
         g.dem_dt = p.dem_dt;
         g.substeps = (int)(p.dt / p.dem_dt);   // source/DEM-MPH.cpp:434
         g.compensated = o.compensated;
+        g.lanes[0] = g.lanes[1] = (o.lanes == 4 || o.lanes == 16) ? o.lanes : 8;
         g.sort_reference = o.sort_mode == DMPH_SORT_REFERENCE;
         g.rebuild_every = o.rebuild == DMPH_REBUILD_EVERY_STEP;
         weights(g, p);
@@ -576,6 +578,69 @@ class Engine final : public dmph_handle {
         return DMPH_OK;
     }
 
+    // FP32 sweeps with 4, 8 and 16 lanes per particle, timed on the current
+    // state by re-running them (they recompute the same values; the forces,
+    // which the DEM pass adds to, are saved and restored), fastest kept.
+    int autotune(int reps, int *lanes) override {
+        HCHK(hipSetDevice(device));
+        if (std::is_same<R, double>::value) {
+            if (lanes) lanes[0] = lanes[1] = 1;
+            return DMPH_OK;
+        }
+        reps = reps < 1 ? 1 : reps;
+        const size_t n = g.n;
+        R *save = a.t_real;   // rebin scratch, free between steps
+        HCHK(hipMemcpyAsync(save, a.fx, n * sizeof(R), hipMemcpyDeviceToDevice, stream));
+        HCHK(hipMemcpyAsync(save + n, a.fy, n * sizeof(R), hipMemcpyDeviceToDevice, stream));
+        HCHK(hipMemcpyAsync(save + 2 * n, a.fz, n * sizeof(R), hipMemcpyDeviceToDevice, stream));
+        hipEvent_t e0 = take_event(), e1 = take_event();
+        if (!e0 || !e1) { err = "hipEventCreate failed"; return DMPH_E_DEVICE; }
+        const int cand[3] = {4, 8, 16};
+        const int orig[2] = {g.lanes[0], g.lanes[1]};
+        int picks[2] = {orig[0], orig[1]};
+        for (int w = 0; w < 2; w++) {
+            float best = 1e30f;
+            int pick = 8;
+            for (int c = 0; c < 3; c++) {
+                g.lanes[w] = cand[c];
+                auto launch = [&] {
+                    if (w == 0) launch_sweep1<R>(g, a, stream);
+                    else launch_sweep2<R>(g, a, stream);
+                };
+                launch();
+                HCHK(hipEventRecord(e0, stream));
+                for (int r = 0; r < reps; r++) launch();
+                HCHK(hipEventRecord(e1, stream));
+                HCHK(hipEventSynchronize(e1));
+                float ms = 0.f;
+                HCHK(hipEventElapsedTime(&ms, e0, e1));
+                tune_ms[w][c] = ms / reps;
+                if (ms < best) { best = ms; pick = cand[c]; }
+            }
+            picks[w] = pick;
+            g.lanes[w] = orig[w];
+        }
+        // the sweeps' outputs as the state's own lane counts computed them (the
+        // candidates sum in other orders), then the forces as the DEM left them
+        launch_sweep1<R>(g, a, stream);
+        launch_sweep2<R>(g, a, stream);
+        g.lanes[0] = picks[0];
+        g.lanes[1] = picks[1];
+        ev_pool.push_back(e0);
+        ev_pool.push_back(e1);
+        HCHK(hipMemcpyAsync(a.fx, save, n * sizeof(R), hipMemcpyDeviceToDevice, stream));
+        HCHK(hipMemcpyAsync(a.fy, save + n, n * sizeof(R), hipMemcpyDeviceToDevice, stream));
+        HCHK(hipMemcpyAsync(a.fz, save + 2 * n, n * sizeof(R), hipMemcpyDeviceToDevice, stream));
+        HCHK(hipStreamSynchronize(stream));
+        HCHK(hipGetLastError());
+        // launch arguments changed: capture the step again at the next call
+        if (graph_exec) { (void)hipGraphExecDestroy(graph_exec); graph_exec = nullptr; }
+        if (graph) { (void)hipGraphDestroy(graph); graph = nullptr; }
+        if (lanes) { lanes[0] = g.lanes[0]; lanes[1] = g.lanes[1]; }
+        return DMPH_OK;
+    }
+    float tune_ms[2][3] = {{0, 0, 0}, {0, 0, 0}};
+
     int step(int nsteps) override {
         HCHK(hipSetDevice(device));
         // per-group event timing needs the launches one by one: eager then
@@ -792,6 +857,8 @@ class Engine final : public dmph_handle {
         out->wall_begin = g.wb; out->wall_end = g.we;
         out->n0a = g.n0a; out->n0p = g.n0p; out->max_radius = g.rmax;
         out->nscenes = g.nscene;
+        out->lanes[0] = std::is_same<R, double>::value ? 1 : g.lanes[0];
+        out->lanes[1] = std::is_same<R, double>::value ? 1 : g.lanes[1];
         Scalars sc;
         HCHK(hipMemcpyAsync(&sc, a.sc, sizeof sc, hipMemcpyDeviceToHost, stream));
         HCHK(hipStreamSynchronize(stream));
@@ -994,6 +1061,7 @@ int dmph_create_batch(const dmph_params *p, const dmph_options *opt, const dmph_
 
 int dmph_step(dmph_handle *h, int nsteps) { return h && nsteps >= 0 ? h->step(nsteps) : DMPH_E_ARG; }
 int dmph_sync(dmph_handle *h) { return h ? h->sync() : DMPH_E_ARG; }
+int dmph_autotune(dmph_handle *h, int reps, int *lanes) { return h ? h->autotune(reps, lanes) : DMPH_E_ARG; }
 int dmph_set_field(dmph_handle *h, int field, int order, const void *src) {
     return h ? h->set_field(field, order, src) : DMPH_E_ARG;
 }

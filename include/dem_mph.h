@@ -73,6 +73,7 @@ typedef struct dmph_options {
     int device;         /* HIP device ordinal */
     int compensated;    /* FP32 only: pair sums carried in FP64 accumulators (1, default) or FP32 (0) */
     int graph;          /* 1 (default): each step replays one captured hipGraph; 0: launches one by one */
+    int lanes;          /* FP32 only: lanes per particle in the sweeps, 4, 8 or 16 (0: 8; see dmph_autotune) */
 } dmph_options;
 
 void dmph_default_options(dmph_options *opt);
@@ -168,6 +169,7 @@ typedef struct dmph_info {
     double cell_width[3], n0a, n0p, max_radius;
     int64_t rebuilds;   /* binning passes so far (device counter) */
     int nscenes;        /* scenes in the handle (dmph_create_batch), 1 otherwise; cells is per scene */
+    int lanes[2];       /* FP32 sweeps 1 and 2: lanes per particle (dmph_autotune); 1 in FP64 */
 } dmph_info;
 int dmph_get_info(dmph_handle *h, dmph_info *info);
 
@@ -201,6 +203,15 @@ enum {
 };
 int dmph_set_timing(dmph_handle *h, int enable);
 int dmph_get_timing(dmph_handle *h, int group, double *total_ms, int64_t *launches);
+
+/* Autotuner (no reference counterpart): times the FP32 sweeps with 4, 8 and
+ * 16 lanes per particle on the handle's current scene(s) -- `reps` launches
+ * per candidate, re-running the sweeps on the current state (they recompute the
+ * same values; forces are restored), so the simulation does not advance -- and
+ * keeps the fastest for each sweep; the step graph is captured again.  The
+ * choice changes FP32 summation order only.  FP64 handles keep one thread per
+ * particle.  Writes the choice to lanes[2] if not NULL. */
+int dmph_autotune(dmph_handle *h, int reps, int *lanes);
 
 const char *dmph_last_error(const dmph_handle *h);
 void dmph_destroy(dmph_handle *h);

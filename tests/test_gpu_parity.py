@@ -107,9 +107,9 @@ def test_fp64_canonical_order_matches_oracle(name):
     assert max(errs.values()) <= 1e-9, errs
 
 
-def _fp32_vs_oracle(name, steps, compensated=True):
+def _fp32_vs_oracle(name, steps, compensated=True, lanes=0):
     g = G.load(name)
-    s = _solver(g, precision=D.PREC_FP32, sort_mode=D.SORT_CANONICAL, compensated=compensated)
+    s = _solver(g, precision=D.PREC_FP32, sort_mode=D.SORT_CANONICAL, compensated=compensated, lanes=lanes)
     o = P.Oracle(P.params_from_dict(G.params(g)), G.scene(g), sort_mode=P.SORT_CANONICAL)
     # binning happens in FP64 on the first call: identical keys and permutation
     np.testing.assert_array_equal(s.field("key", D.ORDER_SLOT), o.get("key"))
@@ -118,16 +118,19 @@ def _fp32_vs_oracle(name, steps, compensated=True):
     return g, s, o
 
 
-@pytest.mark.parametrize("compensated", [True, False], ids=["fp64sums", "fp32sums"])
-@pytest.mark.parametrize("name", ["mix3d", "st3d", "mix2d", "st2d", "wall2d"])
-def test_fp32_one_step_within_stated_tolerance(name, compensated):
-    """FP32 pairs on cell-relative positions, one step, with FP64 or FP32 sums.
+@pytest.mark.parametrize("name,compensated,lanes", [(n, c, 8) for n in ("mix3d", "st3d", "mix2d", "st2d", "wall2d")
+                                                   for c in (True, False)] +
+                         [(n, True, l) for n in ("mix3d", "st3d", "mix2d") for l in (4, 16)])
+def test_fp32_one_step_within_stated_tolerance(name, compensated, lanes):
+    """FP32 pairs on cell-relative positions, one step, with FP64 or FP32 sums
+    and 4, 8 or 16 lanes per particle.
     Stated tolerances (norm-relative, ||x - x64|| / ||x64|| over the field):
       number density sum_j w_p (= VolStrainP + N0p)     1e-6 (per particle: 1e-5 relative)
       fluid acceleration F/m, solid contact force        1e-4
       positions                                           8 ulp of a cell-relative FP32 offset
                                                           (2^-21 x CellWidth: 3e-8 m for 0.065 m cells)"""
-    g, s, o = _fp32_vs_oracle(name, 1, compensated)
+    g, s, o = _fp32_vs_oracle(name, 1, compensated, lanes)
+    assert tuple(s.info().lanes) == (lanes, lanes)
     n0p = o.scalars()["n0p"]
     nd_ref = o.field("vol_strain_p") + n0p
     nd = s.field("vol_strain_p") + n0p
@@ -142,7 +145,7 @@ def test_fp32_one_step_within_stated_tolerance(name, compensated):
     dx = s.field("pos") - o.field("pos")
     L = g["domain_max"] - g["domain_min"]
     dx = dx - L * np.round(dx / L)
-    _report("%s-%s" % (name, "fp64sums" if compensated else "fp32sums"),
+    _report("%s-%s-%d" % (name, "fp64sums" if compensated else "fp32sums", lanes),
             {"density_per_particle": per, "fluid_accel": acc_err, "solid_force": sf_err,
              "pos_abs": np.abs(dx).max()})
     assert per <= 1e-5
@@ -297,4 +300,29 @@ def test_set_field_positions_and_velocities(prec):
     key = (c[:, 0] * cc[1] + c[:, 1]) * cc[2] + c[:, 2]
     agree = np.mean(key == s.field("key")[fl])
     assert agree == 1.0 if prec == D.PREC_FP64 else agree > 0.999
+
+
+def test_autotune_keeps_state_and_tolerance():
+    """dmph_autotune picks 4, 8 or 16 lanes per sweep without advancing or
+    changing the state; the next steps stay within the FP32 tolerances."""
+    g = G.load("mix3d")
+    s = _solver(g, precision=D.PREC_FP32)
+    o = P.Oracle(P.params_from_dict(G.params(g)), G.scene(g), sort_mode=P.SORT_CANONICAL)
+    s.step(2)
+    o.step(2)
+    fields = ("pos", "vel", "force", "pressure_p", "vol_strain_p", "drag_c", "alpha")
+    before = {f: s.field(f) for f in fields}
+    t0 = s.time()
+    lanes = s.autotune(3)
+    assert set(lanes) <= {4, 8, 16} and tuple(s.info().lanes) == lanes
+    assert s.time() == t0
+    for f in fields:
+        np.testing.assert_array_equal(s.field(f), before[f], err_msg=f)
+    s.step(1)
+    o.step(1)
+    typ = o.field("type")
+    fl = typ < 2
+    F, Fr, m = s.field("force"), o.field("force"), o.field("mass")
+    acc = np.linalg.norm((F[fl] - Fr[fl]) / m[fl, None]) / np.linalg.norm(Fr[fl] / m[fl, None])
+    assert acc <= 1e-4
 
