@@ -274,7 +274,7 @@ class Engine final : public dmph_handle {
         g.dem_dt = p.dem_dt;
         g.substeps = (int)(p.dt / p.dem_dt);   // source/DEM-MPH.cpp:434
         g.compensated = o.compensated;
-        g.lanes[0] = g.lanes[1] = (o.lanes == 4 || o.lanes == 16) ? o.lanes : 8;
+        g.lanes[0] = g.lanes[1] = (o.lanes == 2 || o.lanes == 4 || o.lanes == 16) ? o.lanes : 8;
         g.sort_reference = o.sort_mode == DMPH_SORT_REFERENCE;
         g.rebuild_every = o.rebuild == DMPH_REBUILD_EVERY_STEP;
         weights(g, p);
@@ -578,7 +578,7 @@ class Engine final : public dmph_handle {
         return DMPH_OK;
     }
 
-    // FP32 sweeps with 4, 8 and 16 lanes per particle, timed on the current
+    // FP32 sweeps with 2, 4, 8 and 16 lanes per particle, timed on the current
     // state by re-running them (they recompute the same values; the forces,
     // which the DEM pass adds to, are saved and restored), fastest kept.
     int autotune(int reps, int *lanes) override {
@@ -595,13 +595,13 @@ class Engine final : public dmph_handle {
         HCHK(hipMemcpyAsync(save + 2 * n, a.fz, n * sizeof(R), hipMemcpyDeviceToDevice, stream));
         hipEvent_t e0 = take_event(), e1 = take_event();
         if (!e0 || !e1) { err = "hipEventCreate failed"; return DMPH_E_DEVICE; }
-        const int cand[3] = {4, 8, 16};
+        const int cand[4] = {2, 4, 8, 16};
         const int orig[2] = {g.lanes[0], g.lanes[1]};
         int picks[2] = {orig[0], orig[1]};
         for (int w = 0; w < 2; w++) {
             float best = 1e30f;
             int pick = 8;
-            for (int c = 0; c < 3; c++) {
+            for (int c = 0; c < 4; c++) {
                 g.lanes[w] = cand[c];
                 auto launch = [&] {
                     if (w == 0) launch_sweep1<R>(g, a, stream);
@@ -639,7 +639,7 @@ class Engine final : public dmph_handle {
         if (lanes) { lanes[0] = g.lanes[0]; lanes[1] = g.lanes[1]; }
         return DMPH_OK;
     }
-    float tune_ms[2][3] = {{0, 0, 0}, {0, 0, 0}};
+    float tune_ms[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
 
     int step(int nsteps) override {
         HCHK(hipSetDevice(device));

@@ -73,7 +73,7 @@ typedef struct dmph_options {
     int device;         /* HIP device ordinal */
     int compensated;    /* FP32 only: pair sums carried in FP64 accumulators (1, default) or FP32 (0) */
     int graph;          /* 1 (default): each step replays one captured hipGraph; 0: launches one by one */
-    int lanes;          /* FP32 only: lanes per particle in the sweeps, 4, 8 or 16 (0: 8; see dmph_autotune) */
+    int lanes;          /* FP32 only: lanes per particle in the sweeps, 2, 4, 8 or 16 (0: 8; see dmph_autotune) */
 } dmph_options;
 
 void dmph_default_options(dmph_options *opt);
@@ -204,8 +204,8 @@ enum {
 int dmph_set_timing(dmph_handle *h, int enable);
 int dmph_get_timing(dmph_handle *h, int group, double *total_ms, int64_t *launches);
 
-/* Autotuner (no reference counterpart): times the FP32 sweeps with 4, 8 and
- * 16 lanes per particle on the handle's current scene(s) -- `reps` launches
+/* Autotuner (no reference counterpart): times the FP32 sweeps with 2, 4, 8
+ * and 16 lanes per particle on the handle's current scene(s) -- `reps` launches
  * per candidate, re-running the sweeps on the current state (they recompute the
  * same values; forces are restored), so the simulation does not advance -- and
  * keeps the fastest for each sweep; the step graph is captured again.  The

@@ -120,10 +120,10 @@ def _fp32_vs_oracle(name, steps, compensated=True, lanes=0):
 
 @pytest.mark.parametrize("name,compensated,lanes", [(n, c, 8) for n in ("mix3d", "st3d", "mix2d", "st2d", "wall2d")
                                                    for c in (True, False)] +
-                         [(n, True, l) for n in ("mix3d", "st3d", "mix2d") for l in (4, 16)])
+                         [(n, True, l) for n in ("mix3d", "st3d", "mix2d") for l in (2, 4, 16)])
 def test_fp32_one_step_within_stated_tolerance(name, compensated, lanes):
     """FP32 pairs on cell-relative positions, one step, with FP64 or FP32 sums
-    and 4, 8 or 16 lanes per particle.
+    and 2, 4, 8 or 16 lanes per particle.
     Stated tolerances (norm-relative, ||x - x64|| / ||x64|| over the field):
       number density sum_j w_p (= VolStrainP + N0p)     1e-6 (per particle: 1e-5 relative)
       fluid acceleration F/m, solid contact force        1e-4
@@ -303,7 +303,7 @@ def test_set_field_positions_and_velocities(prec):
 
 
 def test_autotune_keeps_state_and_tolerance():
-    """dmph_autotune picks 4, 8 or 16 lanes per sweep without advancing or
+    """dmph_autotune picks 2, 4, 8 or 16 lanes per sweep without advancing or
     changing the state; the next steps stay within the FP32 tolerances."""
     g = G.load("mix3d")
     s = _solver(g, precision=D.PREC_FP32)
@@ -314,7 +314,7 @@ def test_autotune_keeps_state_and_tolerance():
     before = {f: s.field(f) for f in fields}
     t0 = s.time()
     lanes = s.autotune(3)
-    assert set(lanes) <= {4, 8, 16} and tuple(s.info().lanes) == lanes
+    assert set(lanes) <= {2, 4, 8, 16} and tuple(s.info().lanes) == lanes
     assert s.time() == t0
     for f in fields:
         np.testing.assert_array_equal(s.field(f), before[f], err_msg=f)
