@@ -38,7 +38,7 @@ EXPORTED = ("dmph_default_options", "dmph_params_read", "dmph_grid_read", "dmph_
             "dmph_create_batch", "dmph_step", "dmph_sync", "dmph_time", "dmph_get_field", "dmph_set_field",
             "dmph_get_bodies", "dmph_get_info",
             "dmph_virial", "dmph_pairs", "dmph_write_prof", "dmph_write_vtk", "dmph_set_timing", "dmph_get_timing",
-            "dmph_autotune",
+            "dmph_autotune", "dmph_export_state",
             "dmph_last_error", "dmph_destroy")
 
 TIMING_GROUPS = ("predict", "rebin", "sweep1", "sweep2", "accel", "dem", "move", "end")
@@ -269,6 +269,10 @@ class Solver:
         i = Info()
         self._check(lib().dmph_get_info(self.h, C.byref(i)), "dmph_get_info")
         return i
+
+    def export_state(self, pos_ptr: int, vel_ptr: int) -> None:
+        """FP32 positions / velocities [n][3], original order, into device memory at the given addresses."""
+        self._check(lib().dmph_export_state(self.h, C.c_void_p(pos_ptr), C.c_void_p(vel_ptr)), "dmph_export_state")
 
     def autotune(self, reps: int = 5) -> tuple:
         """Pick lanes per particle for the FP32 sweeps by timing; returns (sweep1, sweep2)."""

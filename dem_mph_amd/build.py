@@ -18,6 +18,7 @@ ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 BUILD = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "libdemmph.so")
+NN_LIB = os.path.join(PKG, "libdemmph_nn.so")
 CLI = os.path.join(PKG, "DEM-MPH")
 ARCH = os.environ.get("DMPH_OFFLOAD_ARCH", "gfx950")
 
@@ -56,6 +57,9 @@ def build(verbose: bool = False) -> str:
          os.path.join(CSRC, "dmph_host.cpp"), "-o", os.path.join(BUILD, "dmph_host.o")],
         [hipcc, "-x", "c++", *COMMON, "-ffp-contract=off", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include", "-c",
          os.path.join(CSRC, "dmph_io.cpp"), "-o", os.path.join(BUILD, "dmph_io.o")],
+        # surrogate network kernels (bf16 MFMA), a library of their own
+        [hipcc, "-x", "hip", *COMMON, *DEVICE, "-ffp-contract=fast", "-c", os.path.join(CSRC, "nn", "dmph_nn.hip"),
+         "-o", os.path.join(BUILD, "dmph_nn.o")],
     ]
     with ThreadPoolExecutor(max_workers=4) as ex:
         for r in ex.map(_run, jobs):
@@ -65,6 +69,8 @@ def build(verbose: bool = False) -> str:
     tmp = LIB + ".tmp"
     _run([hipcc, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp, *objs])
     os.replace(tmp, LIB)
+    _run([hipcc, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", NN_LIB + ".tmp", os.path.join(BUILD, "dmph_nn.o")])
+    os.replace(NN_LIB + ".tmp", NN_LIB)
     _run([hipcc, "-x", "c++", *COMMON, "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include", os.path.join(CSRC, "dmph_cli.cpp"), "-o", CLI + ".tmp",
           "-L" + PKG, "-ldemmph", "-Wl,-rpath,$ORIGIN"])
     os.replace(CLI + ".tmp", CLI)

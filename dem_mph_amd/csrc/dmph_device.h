@@ -107,6 +107,7 @@ template <class R> void launch_dem(const Geo &g, const Dev<R> &a, hipStream_t s)
 template <class R> void launch_move(const Geo &g, const Dev<R> &a, hipStream_t s);
 template <class R> void launch_virial(const Geo &g, const Dev<R> &a, hipStream_t s);
 template <class R> void launch_init_density(const Geo &g, const Dev<R> &a, hipStream_t s);
+template <class R> void launch_export(const Geo &g, const Dev<R> &a, float *pos, float *vel, hipStream_t s);
 
 // FP32 sweeps (dmph_sweep32.inc, in the FP32 object only)
 void launch_sweep1_f32(const Geo &g, const Dev<float> &a, hipStream_t s, int init);

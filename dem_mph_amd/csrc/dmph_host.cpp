@@ -28,6 +28,7 @@ struct dmph_handle {
     virtual int time(double *t) = 0;
     virtual int get_field(int field, int order, void *dst) = 0;
     virtual int set_field(int field, int order, const void *src) = 0;
+    virtual int export_state(float *pos, float *vel) = 0;
     virtual int get_bodies(double *x, double *v, double *q, double *l, double *m) = 0;
     virtual int info(dmph_info *out) = 0;
     virtual int virial() = 0;
@@ -835,6 +836,14 @@ class Engine final : public dmph_handle {
         return DMPH_OK;
     }
 
+    int export_state(float *pos, float *vel) override {
+        HCHK(hipSetDevice(device));
+        launch_export<R>(g, a, pos, vel, stream);
+        HCHK(hipGetLastError());
+        HCHK(hipStreamSynchronize(stream));
+        return DMPH_OK;
+    }
+
     int get_bodies(double *x, double *v, double *q, double *l, double *m) override {
         HCHK(hipSetDevice(device));
         const size_t nb = g.nb;
@@ -1061,6 +1070,9 @@ int dmph_create_batch(const dmph_params *p, const dmph_options *opt, const dmph_
 
 int dmph_step(dmph_handle *h, int nsteps) { return h && nsteps >= 0 ? h->step(nsteps) : DMPH_E_ARG; }
 int dmph_sync(dmph_handle *h) { return h ? h->sync() : DMPH_E_ARG; }
+int dmph_export_state(dmph_handle *h, float *pos, float *vel) {
+    return h && (pos || vel) ? h->export_state(pos, vel) : DMPH_E_ARG;
+}
 int dmph_autotune(dmph_handle *h, int reps, int *lanes) { return h ? h->autotune(reps, lanes) : DMPH_E_ARG; }
 int dmph_set_field(dmph_handle *h, int field, int order, const void *src) {
     return h ? h->set_field(field, order, src) : DMPH_E_ARG;

@@ -1,0 +1,776 @@
+// Message-passing surrogate for the DEM-MPH step: HIP kernels for gfx950.
+//
+// Data layout: node-major rows of features, bf16 activations (raw 16-bit
+// storage, FP32 arithmetic), FP32 master weights and gradients.  A weight W of
+// a layer with K inputs and N outputs is stored [N][K] (row n contiguous over
+// k), so the forward GEMM and, with the transposed copy [K][N], the input
+// gradient GEMM read their B operand contiguously along the reduction.
+//
+// Kernels:
+//   gemm      Y = act(X W^T + b [+ R]) on bf16 MFMA (16x16x16 or gfx950 16x16x32),
+//             FP32 accumulate, one 128-column block of Y per workgroup column
+//   gemm_dw   dW = dY^T X and db = sum dY: per-workgroup FP32 partials over a
+//             chunk of rows, summed by reduce_rows (deterministic, no atomics)
+//   edge_fwd  S_i = sum_j relu(A_i + B_j + Wc eps_ij + bc) over the radius graph
+//   edge_bwd  dA, dB (from the symmetric graph: no atomics), dWc, dbc partials
+//   graph     uniform-grid counting sort + CSR radius graph, deterministic order
+//   features, head, loss, Adam, ReLU backward, bf16 casts
+//
+// All entry points are extern "C" with plain device pointers and a stream.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "dem_mph_nn.h"
+
+namespace nn {
+
+typedef uint16_t bf16_t;   // storage
+
+__device__ __forceinline__ float bf2f(bf16_t v) { return __uint_as_float(((uint32_t)v) << 16); }
+// round to nearest even
+__device__ __forceinline__ bf16_t f2bf(float f) {
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7f800000u) == 0x7f800000u) return (bf16_t)(u >> 16 | ((u & 0xffffu) ? 0x40u : 0u));   // inf / nan
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (bf16_t)(u >> 16);
+}
+
+typedef short s4_t __attribute__((ext_vector_type(4)));
+typedef __bf16 b8_t __attribute__((ext_vector_type(8)));
+typedef float f4_t __attribute__((ext_vector_type(4)));
+
+// ---------------------------------------------------------------------------------------------
+// GEMM: Y[M x ldy-strided][n0 .. n0+128) = act(X[M x K] W[N][K]^T + b [+ R])
+// Workgroup: 4 waves, 128 rows; each wave 32 rows x 128 columns = 2 x 8 MFMA
+// tiles of 16x16.  W's 128-row block sits in LDS (K <= 256: at most 64 KB).
+// KS: reduction step of the instruction, 16 (v_mfma_f32_16x16x16_bf16) or 32
+// (v_mfma_f32_16x16x32_bf16, gfx950).  Lane l of a 16x16xKS tile holds A row
+// l % 16, k = (KS/4) * (l / 16) + [0, KS/4); B column l % 16, same k; D column
+// l % 16, rows 4 * (l / 16) + [0, 4).
+// ---------------------------------------------------------------------------------------------
+
+constexpr int kRows = 128;
+constexpr int kCols = 128;
+
+// LDS position of W[r][k]: 16-byte granule k / 8 of row r is stored at granule
+// (k / 8) ^ (r & m), m = min(K / 8, 8) - 1, so the rows an MFMA fragment reads
+// at one k fall in different banks; the XOR never leaves the row.
+__device__ __forceinline__ int wl_at(int r, int k, int K) {
+    const int m = (K >= 64 ? 8 : K / 8) - 1;
+    return r * K + (((k >> 3) ^ (r & m)) << 3) + (k & 7);
+}
+
+template <int KS>
+__global__ void __launch_bounds__(256) k_gemm(const bf16_t *__restrict__ X, int ldx, const bf16_t *__restrict__ W,
+                                              const float *__restrict__ bias, const bf16_t *__restrict__ R, int ldr,
+                                              bf16_t *__restrict__ Y, int ldy, int M, int K, int relu) {
+    extern __shared__ bf16_t wl[];   // [128][K], 16-byte granules XOR-swizzled by row (wl_at)
+    const int n0 = blockIdx.y * kCols;
+    for (int t = threadIdx.x; t < kCols * K / 8; t += blockDim.x) {   // 16-byte copies
+        const int r = t / (K / 8), c = (t % (K / 8)) * 8;
+        *reinterpret_cast<uint4 *>(wl + wl_at(r, c, K)) = *reinterpret_cast<const uint4 *>(W + (size_t)(n0 + r) * K + c);
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int row0 = blockIdx.x * kRows + wave * 32;
+    const int lr = lane & 15, lk = (lane >> 4) * (KS / 4);
+    f4_t acc[2][8];
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int c = 0; c < 8; c++) acc[a][c] = f4_t{0.f, 0.f, 0.f, 0.f};
+    for (int k = 0; k < K; k += KS) {
+        // A fragments of the wave's two 16-row blocks (rows past M read as zero)
+        if constexpr (KS == 16) {
+            s4_t af[2];
+#pragma unroll
+            for (int a = 0; a < 2; a++) {
+                const int r = row0 + a * 16 + lr;
+                if (r < M) {
+                    const uint2 v = *reinterpret_cast<const uint2 *>(X + (size_t)r * ldx + k + lk);
+                    af[a] = s4_t{(short)(v.x & 0xffff), (short)(v.x >> 16), (short)(v.y & 0xffff), (short)(v.y >> 16)};
+                } else {
+                    af[a] = s4_t{0, 0, 0, 0};
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < 8; c++) {
+                const uint2 v = *reinterpret_cast<const uint2 *>(wl + wl_at(c * 16 + lr, k + lk, K));
+                const s4_t bf = s4_t{(short)(v.x & 0xffff), (short)(v.x >> 16), (short)(v.y & 0xffff), (short)(v.y >> 16)};
+#pragma unroll
+                for (int a = 0; a < 2; a++) acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af[a], bf, acc[a][c], 0, 0, 0);
+            }
+        } else {
+            b8_t af[2];
+#pragma unroll
+            for (int a = 0; a < 2; a++) {
+                const int r = row0 + a * 16 + lr;
+                uint4 v = make_uint4(0, 0, 0, 0);
+                if (r < M) v = *reinterpret_cast<const uint4 *>(X + (size_t)r * ldx + k + lk);
+                af[a] = __builtin_bit_cast(b8_t, v);
+            }
+#pragma unroll
+            for (int c = 0; c < 8; c++) {
+                const uint4 v = *reinterpret_cast<const uint4 *>(wl + wl_at(c * 16 + lr, k + lk, K));
+                const b8_t bf = __builtin_bit_cast(b8_t, v);
+#pragma unroll
+                for (int a = 0; a < 2; a++) acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[a], bf, acc[a][c], 0, 0, 0);
+            }
+        }
+    }
+    // epilogue: bias, residual, ReLU, bf16 store
+#pragma unroll
+    for (int c = 0; c < 8; c++) {
+        const int col = n0 + c * 16 + lr;
+        const float b = bias ? bias[col] : 0.f;
+#pragma unroll
+        for (int a = 0; a < 2; a++)
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const int r = row0 + a * 16 + (lane >> 4) * 4 + e;
+                if (r >= M) continue;
+                float v = acc[a][c][e] + b;
+                if (R) v += bf2f(R[(size_t)r * ldr + col]);
+                if (relu) v = fmaxf(v, 0.f);
+                Y[(size_t)r * ldy + col] = f2bf(v);
+            }
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// dW[N][K] = dY^T X, db[N] = sum_m dY: workgroup g sums rows [g*chunk, (g+1)*chunk) in
+// 64-row tiles staged transposed in LDS (dYt[n][m], Xt[k][m]) and writes FP32
+// partials P[g][N][K] and Pb[g][N]; reduce_rows adds the partials.  The output
+// block of a workgroup is 128 (n) x K (k <= 256): wave w owns n rows
+// [32w, 32w + 32) as 2 x (K/16) MFMA tiles.
+// ---------------------------------------------------------------------------------------------
+
+constexpr int kTileM = 64;
+
+template <int KT>   // K / 16
+__global__ void __launch_bounds__(256) k_gemm_dw(const bf16_t *__restrict__ dY, int lddy, const bf16_t *__restrict__ X,
+                                                 int ldx, float *__restrict__ P, float *__restrict__ Pb, int M, int chunk) {
+    constexpr int K = KT * 16;
+    __shared__ bf16_t dyt[kCols][kTileM + 4];
+    __shared__ bf16_t xt[K][kTileM + 4];
+    const int n0 = blockIdx.y * kCols;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lr = lane & 15, lk = (lane >> 4) * 4;
+    f4_t acc[2][KT];
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int c = 0; c < KT; c++) acc[a][c] = f4_t{0.f, 0.f, 0.f, 0.f};
+    float bsum = 0.f;   // thread t < 128: column n0 + t
+    const int m_beg = blockIdx.x * chunk, m_end = min(M, m_beg + chunk);
+    for (int m0 = m_beg; m0 < m_end; m0 += kTileM) {
+        __syncthreads();
+        for (int t = threadIdx.x; t < kTileM * kCols; t += blockDim.x) {
+            const int m = t / kCols, n = t % kCols;
+            dyt[n][m] = (m0 + m < m_end) ? dY[(size_t)(m0 + m) * lddy + n0 + n] : (bf16_t)0;
+        }
+        for (int t = threadIdx.x; t < kTileM * K; t += blockDim.x) {
+            const int m = t / K, k = t % K;
+            xt[k][m] = (m0 + m < m_end) ? X[(size_t)(m0 + m) * ldx + k] : (bf16_t)0;
+        }
+        __syncthreads();
+        if (threadIdx.x < kCols)
+            for (int m = 0; m < kTileM; m++) bsum += bf2f(dyt[threadIdx.x][m]);
+#pragma unroll
+        for (int ms = 0; ms < kTileM; ms += 16) {
+            s4_t af[2];
+#pragma unroll
+            for (int a = 0; a < 2; a++) {
+                const uint2 v = *reinterpret_cast<const uint2 *>(&dyt[wave * 32 + a * 16 + lr][ms + lk]);
+                af[a] = s4_t{(short)(v.x & 0xffff), (short)(v.x >> 16), (short)(v.y & 0xffff), (short)(v.y >> 16)};
+            }
+#pragma unroll
+            for (int c = 0; c < KT; c++) {
+                const uint2 v = *reinterpret_cast<const uint2 *>(&xt[c * 16 + lr][ms + lk]);
+                const s4_t bf = s4_t{(short)(v.x & 0xffff), (short)(v.x >> 16), (short)(v.y & 0xffff), (short)(v.y >> 16)};
+#pragma unroll
+                for (int a = 0; a < 2; a++) acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af[a], bf, acc[a][c], 0, 0, 0);
+            }
+        }
+    }
+    const size_t Nall = (size_t)gridDim.y * kCols;
+    float *pg = P + (size_t)blockIdx.x * Nall * K;
+#pragma unroll
+    for (int c = 0; c < KT; c++) {
+#pragma unroll
+        for (int a = 0; a < 2; a++)
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const int n = n0 + wave * 32 + a * 16 + (lane >> 4) * 4 + e;
+                const int k = c * 16 + lr;
+                pg[(size_t)n * K + k] = acc[a][c][e];
+            }
+    }
+    if (Pb && threadIdx.x < kCols) Pb[(size_t)blockIdx.x * Nall + n0 + threadIdx.x] = bsum;
+}
+
+// out[j] (+)= sum_g P[g * stride + j] for short outputs over many partials: one
+// block per output, threads take g = t, t + 256, ..., then a fixed tree (deterministic)
+__global__ void __launch_bounds__(256) k_reduce_cols(const float *__restrict__ P, int groups, size_t stride,
+                                                     float *__restrict__ out, int accumulate) {
+    __shared__ float red[256];
+    const size_t j = blockIdx.x;
+    float s = 0.f;
+    for (int g = threadIdx.x; g < groups; g += blockDim.x) s += P[(size_t)g * stride + j];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[j] = accumulate ? out[j] + red[0] : red[0];
+}
+
+// out[j] (+)= sum_g P[g * stride + j] for j < len (FP32 partials, in group order)
+__global__ void k_reduce_rows(const float *__restrict__ P, int groups, size_t len, size_t stride, float *__restrict__ out,
+                              int accumulate) {
+    const size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= len) return;
+    float s = 0.f;
+    for (int g = 0; g < groups; g++) s += P[(size_t)g * stride + j];
+    out[j] = accumulate ? out[j] + s : s;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Edge messages.  One wave per node i; lane l holds features 2l, 2l+1.
+//   S_i = sum_{j in row i} relu(A_i + B_j + Wc eps_ij + bc),  eps_ij = ((x_j - x_i)/R, |x_j - x_i|/R)
+// AB: [N][256] = [A | B] (bf16), Wc: [128][4] FP32, bc: [128].
+// ---------------------------------------------------------------------------------------------
+
+__device__ __forceinline__ void edge_eps(const float *pos, int i, int j, float inv_r, float e[4]) {
+    const float dx = pos[3 * j] - pos[3 * i], dy = pos[3 * j + 1] - pos[3 * i + 1], dz = pos[3 * j + 2] - pos[3 * i + 2];
+    e[0] = dx * inv_r; e[1] = dy * inv_r; e[2] = dz * inv_r;
+    e[3] = sqrtf(dx * dx + dy * dy + dz * dz) * inv_r;
+}
+
+__global__ void __launch_bounds__(256) k_edge_fwd(const bf16_t *__restrict__ AB, const float *__restrict__ pos,
+                                                  const int *__restrict__ rp, const int *__restrict__ ci,
+                                                  const float *__restrict__ Wc, const float *__restrict__ bc,
+                                                  bf16_t *__restrict__ S, int N, float inv_r) {
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (i >= N) return;
+    const int f = 2 * (threadIdx.x & 63);
+    const float a0 = bf2f(AB[(size_t)i * 256 + f]), a1 = bf2f(AB[(size_t)i * 256 + f + 1]);
+    const float4 w0 = *reinterpret_cast<const float4 *>(Wc + 4 * f), w1 = *reinterpret_cast<const float4 *>(Wc + 4 * f + 4);
+    const float b0 = bc[f], b1 = bc[f + 1];
+    float s0 = 0.f, s1 = 0.f;
+    for (int q = rp[i]; q < rp[i + 1]; q++) {
+        const int j = ci[q];
+        float e[4];
+        edge_eps(pos, i, j, inv_r, e);
+        const uint32_t bj = *reinterpret_cast<const uint32_t *>(AB + (size_t)j * 256 + 128 + f);
+        const float c0 = a0 + bf2f((bf16_t)(bj & 0xffff)) + b0 + w0.x * e[0] + w0.y * e[1] + w0.z * e[2] + w0.w * e[3];
+        const float c1 = a1 + bf2f((bf16_t)(bj >> 16)) + b1 + w1.x * e[0] + w1.y * e[1] + w1.z * e[2] + w1.w * e[3];
+        s0 += fmaxf(c0, 0.f);
+        s1 += fmaxf(c1, 0.f);
+    }
+    *reinterpret_cast<uint32_t *>(S + (size_t)i * 128 + f) = (uint32_t)f2bf(s0) | ((uint32_t)f2bf(s1) << 16);
+}
+
+// dS: [N][128] FP32 (gradient of S).  Outputs dAB [N][256] bf16 and per-block
+// partials of dWc [128][4] and dbc [128] in Pw[block][128 * 5].
+__global__ void __launch_bounds__(256) k_edge_bwd(const bf16_t *__restrict__ AB, const float *__restrict__ pos,
+                                                  const int *__restrict__ rp, const int *__restrict__ ci,
+                                                  const float *__restrict__ Wc, const float *__restrict__ bc,
+                                                  const float *__restrict__ dS, bf16_t *__restrict__ dAB,
+                                                  float *__restrict__ Pw, int N, float inv_r) {
+    __shared__ float red[4][128 * 5];
+    const int wave = threadIdx.x >> 6;
+    const int n = blockIdx.x * 4 + wave;
+    const int f = 2 * (threadIdx.x & 63);
+    const float4 w0 = *reinterpret_cast<const float4 *>(Wc + 4 * f), w1 = *reinterpret_cast<const float4 *>(Wc + 4 * f + 4);
+    const float b0 = bc[f], b1 = bc[f + 1];
+    float gw[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};   // dWc rows f, f+1 (4 each), dbc f, f+1
+    if (n < N) {
+        const float an0 = bf2f(AB[(size_t)n * 256 + f]), an1 = bf2f(AB[(size_t)n * 256 + f + 1]);
+        const float bn0 = bf2f(AB[(size_t)n * 256 + 128 + f]), bn1 = bf2f(AB[(size_t)n * 256 + 128 + f + 1]);
+        const float ds0 = dS[(size_t)n * 128 + f], ds1 = dS[(size_t)n * 128 + f + 1];
+        float da0 = 0.f, da1 = 0.f, db0 = 0.f, db1 = 0.f;
+        for (int q = rp[n]; q < rp[n + 1]; q++) {
+            const int j = ci[q];
+            float e[4];
+            // n as destination of j -> n: mask with A_n + B_j + c(eps_nj)
+            edge_eps(pos, n, j, inv_r, e);
+            const uint32_t bj = *reinterpret_cast<const uint32_t *>(AB + (size_t)j * 256 + 128 + f);
+            const float c0 = an0 + bf2f((bf16_t)(bj & 0xffff)) + b0 + w0.x * e[0] + w0.y * e[1] + w0.z * e[2] + w0.w * e[3];
+            const float c1 = an1 + bf2f((bf16_t)(bj >> 16)) + b1 + w1.x * e[0] + w1.y * e[1] + w1.z * e[2] + w1.w * e[3];
+            const float m0 = c0 > 0.f ? ds0 : 0.f, m1 = c1 > 0.f ? ds1 : 0.f;
+            da0 += m0; da1 += m1;
+            for (int t = 0; t < 4; t++) { gw[t] += m0 * e[t]; gw[4 + t] += m1 * e[t]; }
+            gw[8] += m0; gw[9] += m1;
+            // n as source of n -> j (the graph is symmetric: j's row holds n): mask with A_j + B_n + c(eps_jn)
+            const uint32_t aj = *reinterpret_cast<const uint32_t *>(AB + (size_t)j * 256 + f);
+            const float ej0 = -e[0], ej1 = -e[1], ej2 = -e[2];
+            const float d0 = bf2f((bf16_t)(aj & 0xffff)) + bn0 + b0 + w0.x * ej0 + w0.y * ej1 + w0.z * ej2 + w0.w * e[3];
+            const float d1 = bf2f((bf16_t)(aj >> 16)) + bn1 + b1 + w1.x * ej0 + w1.y * ej1 + w1.z * ej2 + w1.w * e[3];
+            const float2 dsj = *reinterpret_cast<const float2 *>(dS + (size_t)j * 128 + f);
+            db0 += d0 > 0.f ? dsj.x : 0.f;
+            db1 += d1 > 0.f ? dsj.y : 0.f;
+        }
+        *reinterpret_cast<uint32_t *>(dAB + (size_t)n * 256 + f) = (uint32_t)f2bf(da0) | ((uint32_t)f2bf(da1) << 16);
+        *reinterpret_cast<uint32_t *>(dAB + (size_t)n * 256 + 128 + f) = (uint32_t)f2bf(db0) | ((uint32_t)f2bf(db1) << 16);
+    }
+    for (int t = 0; t < 4; t++) { red[wave][4 * f + t] = gw[t]; red[wave][4 * (f + 1) + t] = gw[4 + t]; }
+    red[wave][512 + f] = gw[8];
+    red[wave][512 + f + 1] = gw[9];
+    __syncthreads();
+    for (int t = threadIdx.x; t < 640; t += blockDim.x)
+        Pw[(size_t)blockIdx.x * 640 + t] = red[0][t] + red[1][t] + red[2][t] + red[3][t];
+}
+
+// ---------------------------------------------------------------------------------------------
+// Radius graph on a uniform grid (cell width >= R), deterministic CSR order:
+// rows list neighbours by ascending cell (x-major) then ascending node index.
+// A node's cell key includes its scene (scene * cells + cell), so scenes of a
+// batch never connect.
+// ---------------------------------------------------------------------------------------------
+
+__device__ __forceinline__ void cell_of(const float *pos, int i, const float *lo, float inv_w, const int *cc, int c[3]) {
+    for (int k = 0; k < 3; k++) {
+        int v = (int)floorf((pos[3 * i + k] - lo[k]) * inv_w);
+        c[k] = v < 0 ? 0 : (v >= cc[k] ? cc[k] - 1 : v);
+    }
+}
+
+__global__ void k_g_keys(const float *pos, const int *scene, int N, NnGrid gd, int *key, int *counts) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    int c[3];
+    cell_of(pos, i, gd.lo, gd.inv_w, gd.cc, c);
+    const int k = (scene ? scene[i] : 0) * gd.ncell + (c[0] * gd.cc[1] + c[1]) * gd.cc[2] + c[2];
+    key[i] = k;
+    atomicAdd(&counts[k], 1);
+}
+
+// exclusive scan of n ints into out[0..n] by one 1024-thread block (tiles of 1024)
+__global__ void __launch_bounds__(1024) k_scan1(const int *in, int *out, int n) {
+    __shared__ int ws[16];
+    __shared__ int carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    for (int b = 0; b < n; b += 1024) {
+        const int t = b + threadIdx.x;
+        const int v = t < n ? in[t] : 0;
+        int inc = v;
+        for (int o = 1; o < 64; o <<= 1) {
+            const int y = __shfl_up(inc, o);
+            if (lane >= o) inc += y;
+        }
+        if (lane == 63) ws[wid] = inc;
+        __syncthreads();
+        int wo = 0;
+        for (int w = 0; w < wid; w++) wo += ws[w];
+        const int c0 = carry;
+        if (t < n) out[t] = c0 + wo + inc - v;
+        __syncthreads();
+        if (threadIdx.x == 1023) carry = c0 + wo + inc;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[n] = carry;
+}
+
+__global__ void k_g_scatter(const int *key, const int *start, int *fill, int *order, int N) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const int k = key[i];
+    order[start[k] + atomicAdd(&fill[k], 1)] = i;
+}
+
+// ascending node index inside each cell
+__global__ void k_g_rank(const int *key, const int *start, const int *order, int *sorted, int N) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= N) return;
+    const int i = order[p], k = key[i];
+    int r = 0;
+    for (int q = start[k]; q < start[k + 1]; q++) r += order[q] < i;
+    sorted[start[k] + r] = i;
+}
+
+template <bool FILL>
+__global__ void k_g_rows(const float *pos, const int *key, const int *start, const int *sorted, int N, NnGrid gd,
+                         float r2, int *deg, const int *rp, int *ci) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const int k = key[i];
+    const int sbase = k - k % gd.ncell;
+    const int cell = k % gd.ncell;
+    const int c[3] = {cell / (gd.cc[1] * gd.cc[2]), (cell / gd.cc[2]) % gd.cc[1], cell % gd.cc[2]};
+    const float xi = pos[3 * i], yi = pos[3 * i + 1], zi = pos[3 * i + 2];
+    int cnt = 0, out = FILL ? rp[i] : 0;
+    for (int x = c[0] - 1; x <= c[0] + 1; x++) {
+        if (x < 0 || x >= gd.cc[0]) continue;
+        for (int y = c[1] - 1; y <= c[1] + 1; y++) {
+            if (y < 0 || y >= gd.cc[1]) continue;
+            for (int z = c[2] - 1; z <= c[2] + 1; z++) {
+                if (z < 0 || z >= gd.cc[2]) continue;
+                const int kk = sbase + (x * gd.cc[1] + y) * gd.cc[2] + z;
+                for (int q = start[kk]; q < start[kk + 1]; q++) {
+                    const int j = sorted[q];
+                    if (j == i) continue;
+                    const float dx = pos[3 * j] - xi, dy = pos[3 * j + 1] - yi, dz = pos[3 * j + 2] - zi;
+                    if (dx * dx + dy * dy + dz * dz > r2) continue;
+                    if (FILL) ci[out++] = j;
+                    cnt++;
+                }
+            }
+        }
+    }
+    if (!FILL) deg[i] = cnt;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Node features (bf16 [N][32]): 5 velocities (newest first) x 3 scaled by vscale,
+// class one-hot (fluid, debris, wall), 6 wall-face distances clipped to
+// [0, 1] in units of R (tank interior box [tlo, thi]), zero padding.
+// ---------------------------------------------------------------------------------------------
+
+__global__ void k_features(const float *vel_hist, int N, const int *cls, const float *pos, NnFeat ft, bf16_t *X) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    bf16_t *x = X + (size_t)i * 32;
+    for (int h = 0; h < 5; h++)
+        for (int k = 0; k < 3; k++) x[3 * h + k] = f2bf(vel_hist[((size_t)h * N + i) * 3 + k] * ft.vscale);
+    for (int c = 0; c < 3; c++) x[15 + c] = f2bf(cls[i] == c ? 1.f : 0.f);
+    for (int k = 0; k < 3; k++) {
+        const float lo = (pos[3 * i + k] - ft.tlo[k]) * ft.inv_r, hi = (ft.thi[k] - pos[3 * i + k]) * ft.inv_r;
+        x[18 + 2 * k] = f2bf(fminf(fmaxf(lo, 0.f), 1.f));
+        x[19 + 2 * k] = f2bf(fminf(fmaxf(hi, 0.f), 1.f));
+    }
+    for (int k = 24; k < 32; k++) x[k] = 0;
+}
+
+// head: y[n][c] = sum_f H[n][f] Wd[c][f] + bd[c] (c < 3), FP32 out
+__global__ void k_head(const bf16_t *H, const float *Wd, const float *bd, float *Yo, int N) {
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (i >= N) return;
+    const int f = 2 * (threadIdx.x & 63);
+    const uint32_t h = *reinterpret_cast<const uint32_t *>(H + (size_t)i * 128 + f);
+    const float h0 = bf2f((bf16_t)(h & 0xffff)), h1 = bf2f((bf16_t)(h >> 16));
+    for (int c = 0; c < 3; c++) {
+        float v = h0 * Wd[c * 128 + f] + h1 * Wd[c * 128 + f + 1];
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+        if ((threadIdx.x & 63) == 0) Yo[3 * i + c] = v + bd[c];
+    }
+}
+
+// loss = sum_{mask} |y - t|^2 / count; dY = 2 (y - t) / count on masked nodes (else 0);
+// per-block loss partials in Pl
+__global__ void k_mse(const float *Yo, const float *T, const int *mask, int N, float inv_count, float clip, float *dY,
+                      float *Pl) {
+    __shared__ float red[256];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    float l = 0.f;
+    if (i < N) {
+        for (int c = 0; c < 3; c++) {
+            const float t = fminf(fmaxf(T[3 * i + c], -clip), clip);
+            const float d = (mask[i] ? Yo[3 * i + c] - t : 0.f);
+            l += d * d;
+            dY[3 * i + c] = 2.f * d * inv_count;
+        }
+    }
+    red[threadIdx.x] = l * inv_count;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) Pl[blockIdx.x] = red[0];
+}
+
+// head backward: dH[n][f] = (sum_c dY[n][c] Wd[c][f]) [H > 0] (bf16; H = relu(.)),
+// partials of dWd [3][128] and dbd [3] per block in Pw[block][387]
+__global__ void k_head_bwd(const bf16_t *H, const float *Wd, const float *dY, bf16_t *dH, float *Pw, int N) {
+    __shared__ float red[4][387];
+    const int wave = threadIdx.x >> 6;
+    const int i = blockIdx.x * 4 + wave;
+    const int f = 2 * (threadIdx.x & 63);
+    float g[6] = {0, 0, 0, 0, 0, 0}, yb[3] = {0, 0, 0};
+    if (i < N) {
+        const uint32_t h = *reinterpret_cast<const uint32_t *>(H + (size_t)i * 128 + f);
+        const float h0 = bf2f((bf16_t)(h & 0xffff)), h1 = bf2f((bf16_t)(h >> 16));
+        const float y[3] = {dY[3 * i], dY[3 * i + 1], dY[3 * i + 2]};
+        float d0 = y[0] * Wd[f] + y[1] * Wd[128 + f] + y[2] * Wd[256 + f];
+        float d1 = y[0] * Wd[f + 1] + y[1] * Wd[128 + f + 1] + y[2] * Wd[256 + f + 1];
+        if (!(h0 > 0.f)) d0 = 0.f;
+        if (!(h1 > 0.f)) d1 = 0.f;
+        *reinterpret_cast<uint32_t *>(dH + (size_t)i * 128 + f) = (uint32_t)f2bf(d0) | ((uint32_t)f2bf(d1) << 16);
+        for (int c = 0; c < 3; c++) { g[2 * c] = y[c] * h0; g[2 * c + 1] = y[c] * h1; yb[c] = y[c]; }
+    }
+    for (int c = 0; c < 3; c++) { red[wave][c * 128 + f] = g[2 * c]; red[wave][c * 128 + f + 1] = g[2 * c + 1]; }
+    if (f == 0)
+        for (int c = 0; c < 3; c++) red[wave][384 + c] = yb[c];
+    __syncthreads();
+    for (int t = threadIdx.x; t < 387; t += blockDim.x)
+        Pw[(size_t)blockIdx.x * 387 + t] = red[0][t] + red[1][t] + red[2][t] + red[3][t];
+}
+
+// dX = dY * [Y > 0] (bf16, in place allowed)
+__global__ void k_relu_bwd(const bf16_t *Y, int ldy, const bf16_t *dY, int lddy, bf16_t *dX, int lddx, int M, int n) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)M * n) return;
+    const size_t r = t / n, c = t % n;
+    dX[r * lddx + c] = bf2f(Y[r * ldy + c]) > 0.f ? dY[r * lddy + c] : (bf16_t)0;
+}
+
+// y = x (FP32 -> bf16) and y += x conversions with strides
+__global__ void k_f2bf(const float *x, int ldx, bf16_t *y, int ldy, int M, int n) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)M * n) return;
+    const size_t r = t / n, c = t % n;
+    y[r * ldy + c] = f2bf(x[r * ldx + c]);
+}
+__global__ void k_bf2f_acc(const bf16_t *x, int ldx, float *y, int ldy, int M, int n, int accumulate) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)M * n) return;
+    const size_t r = t / n, c = t % n;
+    const float v = bf2f(x[r * ldx + c]);
+    y[r * ldy + c] = accumulate ? y[r * ldy + c] + v : v;
+}
+
+// Adam on FP32 master weights; writes the bf16 copy [N][K] and its transpose [K][N] if given
+__global__ void k_adam(float *w, const float *g, float *m, float *v, int n, float lr, float b1, float b2, float eps,
+                       float c1, float c2, float wd, bf16_t *wb, bf16_t *wbt, int rows, int cols) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const float gi = g[t] + wd * w[t];
+    const float mi = b1 * m[t] + (1.f - b1) * gi, vi = b2 * v[t] + (1.f - b2) * gi * gi;
+    m[t] = mi;
+    v[t] = vi;
+    const float wi = w[t] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+    w[t] = wi;
+    if (wb) wb[t] = f2bf(wi);
+    if (wbt) wbt[(size_t)(t % cols) * rows + t / cols] = f2bf(wi);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Rollout at stride h: debris bodies take the mean predicted acceleration of their
+// members (translation only), fluid particles their own; walls stay.  Velocity
+// history [5][N][3] shifts by one (newest first).
+// ---------------------------------------------------------------------------------------------
+
+__global__ void k_body_acc(const float *Y, const int *cls, const int *body, int N, float *bsum) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N || cls[i] != 1 || body[i] < 0) return;
+    float *b = bsum + 4 * body[i];
+    atomicAdd(b, Y[3 * i]);
+    atomicAdd(b + 1, Y[3 * i + 1]);
+    atomicAdd(b + 2, Y[3 * i + 2]);
+    atomicAdd(b + 3, 1.f);
+}
+
+__global__ void k_integrate(float *pos, float *vh, const float *Y, float inv_ascale, const int *cls, const int *body,
+                            const float *bsum, float h, int N) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N || cls[i] == 2) return;
+    float a[3];
+    if (cls[i] == 1 && body[i] >= 0) {
+        const float *b = bsum + 4 * body[i];
+        for (int k = 0; k < 3; k++) a[k] = b[k] / b[3] * inv_ascale;
+    } else {
+        for (int k = 0; k < 3; k++) a[k] = Y[3 * i + k] * inv_ascale;
+    }
+    const size_t st = (size_t)N * 3;
+    for (int k = 0; k < 3; k++) {
+        const float v = vh[3 * (size_t)i + k] + h * a[k];
+        for (int t = 4; t > 0; t--) vh[t * st + 3 * (size_t)i + k] = vh[(t - 1) * st + 3 * (size_t)i + k];
+        vh[3 * (size_t)i + k] = v;
+        pos[3 * i + k] += h * v;
+    }
+}
+
+// sum of `groups` partial rows: coalesced over outputs when there are many of
+// them and few partials, one block per output otherwise
+static void reduce_partials(const float *P, int groups, size_t len, size_t stride, float *out, int accumulate,
+                            hipStream_t s) {
+    if (len >= 4096 && groups <= 1024)
+        hipLaunchKernelGGL(k_reduce_rows, dim3((len + 255) / 256), dim3(256), 0, s, P, groups, len, stride, out, accumulate);
+    else
+        hipLaunchKernelGGL(k_reduce_cols, dim3(len), dim3(256), 0, s, P, groups, stride, out, accumulate);
+}
+
+}  // namespace nn
+
+using namespace nn;
+
+static int launch_ok() { return hipGetLastError() == hipSuccess ? 0 : -5; }
+
+extern "C" {
+
+int dmph_nn_gemm(const uint16_t *X, int ldx, const uint16_t *W, const float *bias, const uint16_t *R, int ldr,
+                 uint16_t *Y, int ldy, int M, int K, int N, int relu, int kstep, void *stream) {
+    if (M <= 0) return 0;
+    // K / 8 must be a power of two (the LDS swizzle stays inside a row)
+    if ((K != 32 && K != 64 && K != 128 && K != 256) || N % 128 || (kstep != 16 && kstep != 32)) return -1;
+    const dim3 grid((M + kRows - 1) / kRows, N / kCols);
+    const size_t lds = (size_t)kCols * K * sizeof(bf16_t);   // <= 64 KB
+    if (kstep == 32)
+        hipLaunchKernelGGL(k_gemm<32>, grid, dim3(256), lds, (hipStream_t)stream, X, ldx, W, bias, R, ldr, Y, ldy, M, K, relu);
+    else
+        hipLaunchKernelGGL(k_gemm<16>, grid, dim3(256), lds, (hipStream_t)stream, X, ldx, W, bias, R, ldr, Y, ldy, M, K, relu);
+    return launch_ok();
+}
+
+size_t dmph_nn_dw_scratch(int M, int K, int N, int groups) {
+    (void)M;
+    return (size_t)groups * (size_t)N * (K + 1) * sizeof(float);
+}
+
+int dmph_nn_gemm_dw(const uint16_t *dY, int lddy, const uint16_t *X, int ldx, float *dW, float *db, int M, int K, int N,
+                    int groups, float *scratch, int accumulate, void *stream) {
+    if ((K != 32 && K != 128 && K != 256) || N % 128 || groups <= 0) return -1;
+    hipStream_t s = (hipStream_t)stream;
+    if (M <= 0) return 0;
+    const int chunk = ((M + groups - 1) / groups + kTileM - 1) / kTileM * kTileM;
+    const int g = (M + chunk - 1) / chunk;
+    float *P = scratch, *Pb = scratch + (size_t)g * N * K;
+    const dim3 grid(g, N / kCols);
+    float *pb = db ? Pb : nullptr;
+    switch (K) {
+        case 32: hipLaunchKernelGGL(k_gemm_dw<2>, grid, dim3(256), 0, s, dY, lddy, X, ldx, P, pb, M, chunk); break;
+        case 128: hipLaunchKernelGGL(k_gemm_dw<8>, grid, dim3(256), 0, s, dY, lddy, X, ldx, P, pb, M, chunk); break;
+        case 256: hipLaunchKernelGGL(k_gemm_dw<16>, grid, dim3(256), 0, s, dY, lddy, X, ldx, P, pb, M, chunk); break;
+        default: return -1;
+    }
+    const size_t len = (size_t)N * K;
+    reduce_partials(P, g, len, len, dW, accumulate, s);
+    if (db) reduce_partials(Pb, g, (size_t)N, (size_t)N, db, accumulate, s);
+    return launch_ok();
+}
+
+int dmph_nn_edge_fwd(const uint16_t *AB, const float *pos, const int *rp, const int *ci, const float *Wc,
+                     const float *bc, uint16_t *S, int N, float radius, void *stream) {
+    if (N <= 0) return 0;
+    hipLaunchKernelGGL(k_edge_fwd, dim3((N + 3) / 4), dim3(256), 0, (hipStream_t)stream, AB, pos, rp, ci, Wc, bc, S, N,
+                       1.f / radius);
+    return launch_ok();
+}
+
+size_t dmph_nn_edge_scratch(int N) { return (size_t)((N + 3) / 4) * 640 * sizeof(float); }
+
+int dmph_nn_edge_bwd(const uint16_t *AB, const float *pos, const int *rp, const int *ci, const float *Wc,
+                     const float *bc, const float *dS, uint16_t *dAB, float *dWc, float *dbc, int N, float radius,
+                     float *scratch, int accumulate, void *stream) {
+    if (N <= 0) return 0;
+    hipStream_t s = (hipStream_t)stream;
+    const int g = (N + 3) / 4;
+    hipLaunchKernelGGL(k_edge_bwd, dim3(g), dim3(256), 0, s, AB, pos, rp, ci, Wc, bc, dS, dAB, scratch, N, 1.f / radius);
+    // partial layout per block (640 floats): [128][4] weights, then [128] bias
+    reduce_partials(scratch, g, 512, 640, dWc, accumulate, s);
+    reduce_partials(scratch + 512, g, 128, 640, dbc, accumulate, s);
+    return launch_ok();
+}
+
+int dmph_nn_graph(const float *pos, const int *scene, int N, const NnGrid *grid, float radius, int *key, int *counts,
+                  int *start, int *fill, int *order, int *sorted, int *deg, int *rp, int *ci, int cap, int *nnz_out,
+                  void *stream) {
+    hipStream_t s = (hipStream_t)stream;
+    const int nk = grid->ncell * (grid->nscene > 0 ? grid->nscene : 1);
+    if (hipMemsetAsync(counts, 0, (size_t)(nk + 1) * sizeof(int), s) != hipSuccess) return -5;
+    if (hipMemsetAsync(fill, 0, (size_t)(nk + 1) * sizeof(int), s) != hipSuccess) return -5;
+    const int b = (N + 255) / 256;
+    hipLaunchKernelGGL(k_g_keys, dim3(b), dim3(256), 0, s, pos, scene, N, *grid, key, counts);
+    hipLaunchKernelGGL(k_scan1, dim3(1), dim3(1024), 0, s, counts, start, nk);
+    hipLaunchKernelGGL(k_g_scatter, dim3(b), dim3(256), 0, s, key, start, fill, order, N);
+    hipLaunchKernelGGL(k_g_rank, dim3(b), dim3(256), 0, s, key, start, order, sorted, N);
+    const float r2 = radius * radius;
+    hipLaunchKernelGGL(k_g_rows<false>, dim3(b), dim3(256), 0, s, pos, key, start, sorted, N, *grid, r2, deg, nullptr,
+                       nullptr);
+    hipLaunchKernelGGL(k_scan1, dim3(1), dim3(1024), 0, s, deg, rp, N);
+    int nnz = 0;
+    if (hipMemcpyAsync(&nnz, rp + N, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess) return -5;
+    if (hipStreamSynchronize(s) != hipSuccess) return -5;
+    *nnz_out = nnz;
+    if (nnz > cap) return -2;   // caller grows ci and calls again
+    hipLaunchKernelGGL(k_g_rows<true>, dim3(b), dim3(256), 0, s, pos, key, start, sorted, N, *grid, r2, deg, rp, ci);
+    return launch_ok();
+}
+
+int dmph_nn_features(const float *vel_hist, int N, const int *cls, const float *pos, const NnFeat *ft, uint16_t *X,
+                     void *stream) {
+    if (N <= 0) return 0;
+    hipLaunchKernelGGL(k_features, dim3((N + 255) / 256), dim3(256), 0, (hipStream_t)stream, vel_hist, N, cls, pos, *ft, X);
+    return launch_ok();
+}
+
+int dmph_nn_head(const uint16_t *H, const float *Wd, const float *bd, float *Y, int N, void *stream) {
+    if (N <= 0) return 0;
+    hipLaunchKernelGGL(k_head, dim3((N + 3) / 4), dim3(256), 0, (hipStream_t)stream, H, Wd, bd, Y, N);
+    return launch_ok();
+}
+
+size_t dmph_nn_head_scratch(int N) { return (size_t)((N + 3) / 4) * 387 * sizeof(float); }
+
+int dmph_nn_head_bwd(const uint16_t *H, const float *Wd, const float *dY, uint16_t *dH, float *dWd, float *dbd, int N,
+                     float *scratch, void *stream) {
+    if (N <= 0) return 0;
+    hipStream_t s = (hipStream_t)stream;
+    const int g = (N + 3) / 4;
+    hipLaunchKernelGGL(k_head_bwd, dim3(g), dim3(256), 0, s, H, Wd, dY, dH, scratch, N);
+    reduce_partials(scratch, g, 384, 387, dWd, 0, s);
+    reduce_partials(scratch + 384, g, 3, 387, dbd, 0, s);
+    return launch_ok();
+}
+
+size_t dmph_nn_mse_scratch(int N) { return (size_t)((N + 255) / 256) * sizeof(float); }
+
+int dmph_nn_mse(const float *Y, const float *T, const int *mask, int N, int count, float clip, float *dY, float *loss,
+                float *scratch, void *stream) {
+    if (N <= 0 || count <= 0 || !(clip > 0.f)) return -1;
+    hipStream_t s = (hipStream_t)stream;
+    const int g = (N + 255) / 256;
+    hipLaunchKernelGGL(k_mse, dim3(g), dim3(256), 0, s, Y, T, mask, N, 1.f / (float)count, clip, dY, scratch);
+    reduce_partials(scratch, g, 1, 1, loss, 0, s);
+    return launch_ok();
+}
+
+int dmph_nn_relu_bwd(const uint16_t *Y, int ldy, const uint16_t *dY, int lddy, uint16_t *dX, int lddx, int M, int n,
+                     void *stream) {
+    const size_t t = (size_t)M * n;
+    if (!t) return 0;
+    hipLaunchKernelGGL(k_relu_bwd, dim3((t + 255) / 256), dim3(256), 0, (hipStream_t)stream, Y, ldy, dY, lddy, dX, lddx, M, n);
+    return launch_ok();
+}
+
+int dmph_nn_f2bf(const float *x, int ldx, uint16_t *y, int ldy, int M, int n, void *stream) {
+    const size_t t = (size_t)M * n;
+    if (!t) return 0;
+    hipLaunchKernelGGL(k_f2bf, dim3((t + 255) / 256), dim3(256), 0, (hipStream_t)stream, x, ldx, y, ldy, M, n);
+    return launch_ok();
+}
+
+int dmph_nn_bf2f(const uint16_t *x, int ldx, float *y, int ldy, int M, int n, int accumulate, void *stream) {
+    const size_t t = (size_t)M * n;
+    if (!t) return 0;
+    hipLaunchKernelGGL(k_bf2f_acc, dim3((t + 255) / 256), dim3(256), 0, (hipStream_t)stream, x, ldx, y, ldy, M, n, accumulate);
+    return launch_ok();
+}
+
+int dmph_nn_rollout_step(float *pos, float *vel_hist, const float *Y, float ascale, const int *cls, const int *body,
+                         float *body_sums, int nbodies, float h, int N, void *stream) {
+    if (N <= 0) return 0;
+    hipStream_t s = (hipStream_t)stream;
+    if (nbodies > 0) {
+        if (hipMemsetAsync(body_sums, 0, (size_t)nbodies * 4 * sizeof(float), s) != hipSuccess) return -5;
+        hipLaunchKernelGGL(k_body_acc, dim3((N + 255) / 256), dim3(256), 0, s, Y, cls, body, N, body_sums);
+    }
+    hipLaunchKernelGGL(k_integrate, dim3((N + 255) / 256), dim3(256), 0, s, pos, vel_hist, Y, 1.f / ascale, cls, body,
+                       body_sums, h, N);
+    return launch_ok();
+}
+
+int dmph_nn_adam(float *w, const float *g, float *m, float *v, int n, float lr, float b1, float b2, float eps, int step,
+                 float wd, uint16_t *wb, uint16_t *wbt, int rows, int cols, void *stream) {
+    if (n <= 0) return 0;
+    const float c1 = 1.f - powf(b1, (float)step), c2 = 1.f - powf(b2, (float)step);
+    hipLaunchKernelGGL(k_adam, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, w, g, m, v, n, lr, b1, b2, eps,
+                       c1, c2, wd, wb, wbt, rows, cols);
+    return launch_ok();
+}
+
+}  // extern "C"

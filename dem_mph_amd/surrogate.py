@@ -1,0 +1,629 @@
+"""Learned surrogate for the DEM-MPH step: a message-passing network on the
+radius graph (r <= 3 d) that predicts particle accelerations at a 10 dt stride.
+
+No reference counterpart (the reference has no learned component).  The
+network, per node (particle):
+
+    x0  = features: 5 velocities at 10 dt spacing (newest first, scaled),
+          class one-hot (fluid, debris, wall), 6 wall-face distances clipped
+          to [0, 1] in units of the radius, zero padding to 32
+    h   = We2 relu(We1 x0 + be1) + be2                         (encoder, 2 layers)
+    6 rounds:
+      A | B = Wab h                                            (per node, 256)
+      S_i   = sum_{j in N(i)} relu(A_i + B_j + Wc eps_ij + bc)  (edge layer 1)
+      agg_i = W2 S_i                                           (edge layer 2)
+      h    += Wn2 relu(Wn1 [h | agg] + bn1) + bn2              (node MLP)
+    y   = Wd2 relu(Wd1 h + bd1) + bd2                          (decoder, 3 outputs)
+
+The edge MLP is the 2-layer width-128 MLP m_ij = W2 relu(W1 [h_i, h_j, eps_ij]
++ b) with its first layer split by input (W1 = [Wa | Wb | Wc]) and its linear
+second layer applied after the sum over j -- the same function, with every
+dense product on a per-node bf16 MFMA GEMM and the per-edge work a gather,
+add, ReLU and sum.  The backward pass recomputes each edge's ReLU mask from the
+per-node projections instead of storing per-edge activations.
+
+Torch tensors are used only as device memory; every operation is a kernel of
+libdemmph_nn.so (include/dem_mph_nn.h) called through its C ABI.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Dict, List, Optional, Tuple
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+NN_LIB = os.path.join(PKG, "libdemmph_nn.so")
+
+WIDTH = 128
+ROUNDS = 6
+FEAT = 32
+HIST = 5
+
+
+class NnGrid(C.Structure):
+    _fields_ = [("lo", C.c_float * 3), ("inv_w", C.c_float), ("cc", C.c_int * 3), ("ncell", C.c_int),
+                ("nscene", C.c_int)]
+
+
+class NnFeat(C.Structure):
+    _fields_ = [("vscale", C.c_float), ("tlo", C.c_float * 3), ("thi", C.c_float * 3), ("inv_r", C.c_float)]
+
+
+_nn = None
+
+
+def nnlib():
+    """The surrogate kernels; a missing library is an error (there is no fallback)."""
+    global _nn
+    if _nn is None:
+        if not os.path.exists(NN_LIB):
+            raise RuntimeError("%s is not built (python -m dem_mph_amd.build)" % NN_LIB)
+        _nn = C.CDLL(NN_LIB)
+        for f in ("dmph_nn_dw_scratch", "dmph_nn_edge_scratch", "dmph_nn_head_scratch", "dmph_nn_mse_scratch"):
+            getattr(_nn, f).restype = C.c_size_t
+    return _nn
+
+
+def _p(t) -> C.c_void_p:
+    return C.c_void_p(t.data_ptr()) if t is not None else C.c_void_p(0)
+
+
+def _chk(rc: int, what: str) -> None:
+    if rc != 0:
+        raise RuntimeError("%s failed (%d)" % (what, rc))
+
+
+class Layer:
+    """FP32 master weight [n_out][n_in] (+ bias), bf16 copy and transposed bf16
+    copy for the GEMMs, gradient and Adam moments."""
+
+    def __init__(self, torch, dev, n_out: int, n_in: int, bias: bool, scale: float, rng: np.random.Generator,
+                 bf16: bool = True):
+        w = rng.standard_normal((n_out, n_in)) * scale
+        self.w = torch.tensor(w, dtype=torch.float32, device=dev)
+        self.b = torch.zeros(n_out, dtype=torch.float32, device=dev) if bias else None
+        self.gw = torch.zeros_like(self.w)
+        self.gb = torch.zeros_like(self.b) if bias else None
+        self.mw, self.vw = torch.zeros_like(self.w), torch.zeros_like(self.w)
+        self.mb = torch.zeros_like(self.b) if bias else None
+        self.vb = torch.zeros_like(self.b) if bias else None
+        self.n_out, self.n_in = n_out, n_in
+        self.wb = torch.zeros((n_out, n_in), dtype=torch.int16, device=dev) if bf16 else None
+        self.wbt = torch.zeros((n_in, n_out), dtype=torch.int16, device=dev) if bf16 else None
+
+    def tensors(self) -> Dict[str, object]:
+        d = {"w": self.w}
+        if self.b is not None:
+            d["b"] = self.b
+        return d
+
+
+class Surrogate:
+    """The network, its parameters and the training/inference passes on one GPU."""
+
+    def __init__(self, d: float, tank_lo, tank_hi, vscale: float, ascale: float, seed: int = 0, kstep: int = 32,
+                 device: str = "cuda:0"):
+        import torch
+        self.torch = torch
+        self.dev = torch.device(device)
+        self.d = d
+        self.radius = 3.0 * d
+        self.tank_lo = np.asarray(tank_lo, float)
+        self.tank_hi = np.asarray(tank_hi, float)
+        self.vscale = float(vscale)     # features: v * vscale
+        self.ascale = float(ascale)     # targets: a * ascale
+        self.kstep = kstep
+        rng = np.random.default_rng(seed)
+        T, W = torch, WIDTH
+        he = lambda n_in: np.sqrt(2.0 / n_in)
+        self.enc1 = Layer(T, self.dev, W, FEAT, True, he(FEAT), rng)
+        self.enc2 = Layer(T, self.dev, W, W, True, he(W) * 0.5, rng)
+        self.rounds = []
+        for _ in range(ROUNDS):
+            r = {
+                "ab": Layer(T, self.dev, 2 * W, W, False, he(W) * 0.5, rng),
+                "c": Layer(T, self.dev, W, 4, True, 0.5, rng, bf16=False),
+                "w2": Layer(T, self.dev, W, W, False, he(W) * 0.1, rng),
+                "n1": Layer(T, self.dev, W, 2 * W, True, he(2 * W), rng),
+                "n2": Layer(T, self.dev, W, W, True, he(W) * 0.1, rng),
+            }
+            self.rounds.append(r)
+        self.dec1 = Layer(T, self.dev, W, W, True, he(W), rng)
+        self.dec2 = Layer(T, self.dev, 3, W, True, he(W) * 0.1, rng, bf16=False)
+        self.layers: List[Layer] = [self.enc1, self.enc2] + [l for r in self.rounds for l in r.values()] + \
+                                   [self.dec1, self.dec2]
+        self.adam_step = 0
+        self._refresh_bf16()
+        self.stream = C.c_void_p(torch.cuda.current_stream(self.dev).cuda_stream)
+        self._ws = None
+        self._graph_ws = None
+
+    # ------------------------------------------------------------------ parameters
+    def _refresh_bf16(self) -> None:
+        """bf16 copies of every GEMM weight (an Adam step with lr 0 rewrites them)."""
+        lib = nnlib()
+        s = C.c_void_p(self.torch.cuda.current_stream(self.dev).cuda_stream)
+        for l in self.layers:
+            if l.wb is None:
+                continue
+            z = self.torch.zeros_like(l.w)
+            _chk(lib.dmph_nn_adam(_p(l.w), _p(z), _p(self.torch.zeros_like(l.w)), _p(self.torch.zeros_like(l.w)),
+                                  l.w.numel(), C.c_float(0.0), C.c_float(0.9), C.c_float(0.999), C.c_float(1.0), 1,
+                                  C.c_float(0.0), _p(l.wb), _p(l.wbt), l.n_out, l.n_in, s), "bf16 copy")
+
+    def state_dict(self) -> Dict[str, np.ndarray]:
+        out = {}
+        names = ["enc1", "enc2"] + ["r%d_%s" % (k, n) for k in range(ROUNDS) for n in ("ab", "c", "w2", "n1", "n2")] + \
+                ["dec1", "dec2"]
+        for name, l in zip(names, self.layers):
+            out[name + ".w"] = l.w.detach().cpu().numpy().astype(np.float64)
+            if l.b is not None:
+                out[name + ".b"] = l.b.detach().cpu().numpy().astype(np.float64)
+        return out
+
+    def grads(self) -> Dict[str, np.ndarray]:
+        out = {}
+        names = ["enc1", "enc2"] + ["r%d_%s" % (k, n) for k in range(ROUNDS) for n in ("ab", "c", "w2", "n1", "n2")] + \
+                ["dec1", "dec2"]
+        for name, l in zip(names, self.layers):
+            out[name + ".w"] = l.gw.detach().cpu().numpy().astype(np.float64)
+            if l.gb is not None:
+                out[name + ".b"] = l.gb.detach().cpu().numpy().astype(np.float64)
+        return out
+
+    def load_state(self, sd: Dict[str, np.ndarray]) -> None:
+        names = ["enc1", "enc2"] + ["r%d_%s" % (k, n) for k in range(ROUNDS) for n in ("ab", "c", "w2", "n1", "n2")] + \
+                ["dec1", "dec2"]
+        for name, l in zip(names, self.layers):
+            l.w.copy_(self.torch.tensor(sd[name + ".w"], dtype=self.torch.float32))
+            if l.b is not None:
+                l.b.copy_(self.torch.tensor(sd[name + ".b"], dtype=self.torch.float32))
+        self._refresh_bf16()
+
+    # ------------------------------------------------------------------ graph
+    def graph(self, pos, scene=None, nscene: int = 1) -> Tuple[object, object, int]:
+        """CSR radius graph (r <= 3 d) of the nodes at pos [N, 3] (FP32, device)."""
+        torch = self.torch
+        N = pos.shape[0]
+        lo = pos.min(dim=0).values.detach().cpu().numpy().astype(np.float64) - 1e-6
+        hi = pos.max(dim=0).values.detach().cpu().numpy().astype(np.float64) + 1e-6
+        w = self.radius * (1.0 + 1e-5)            # cubic cells at least the radius wide
+        cc = np.maximum(np.ceil((hi - lo) / w).astype(int), 1)
+        g = NnGrid()
+        for k in range(3):
+            g.lo[k] = float(lo[k])
+            g.cc[k] = int(cc[k])
+        g.inv_w = float(1.0 / w)
+        g.ncell = int(cc.prod())
+        g.nscene = nscene
+        nk = g.ncell * nscene
+        ws = self._graph_ws
+        if ws is None or ws["N"] < N or ws["nk"] < nk:
+            i32 = lambda n: torch.empty(max(n, 1), dtype=torch.int32, device=self.dev)
+            ws = {"N": N, "nk": nk, "key": i32(N), "order": i32(N), "sorted": i32(N), "deg": i32(N),
+                  "counts": i32(nk + 1), "start": i32(nk + 1), "fill": i32(nk + 1), "cap": 0, "ci": None}
+            self._graph_ws = ws
+        rp = torch.empty(N + 1, dtype=torch.int32, device=self.dev)
+        nnz = C.c_int(0)
+        lib = nnlib()
+        stream = C.c_void_p(torch.cuda.current_stream(self.dev).cuda_stream)
+        for _ in range(2):
+            ci = ws["ci"]
+            rc = lib.dmph_nn_graph(_p(pos), _p(scene), N, C.byref(g), C.c_float(self.radius), _p(ws["key"]),
+                                   _p(ws["counts"]), _p(ws["start"]), _p(ws["fill"]), _p(ws["order"]),
+                                   _p(ws["sorted"]), _p(ws["deg"]), _p(rp), _p(ci), ws["cap"], C.byref(nnz), stream)
+            if rc == -2:
+                ws["cap"] = int(nnz.value * 1.25) + 1024
+                ws["ci"] = torch.empty(ws["cap"], dtype=torch.int32, device=self.dev)
+                continue
+            _chk(rc, "dmph_nn_graph")
+            break
+        return rp, ws["ci"], int(nnz.value)
+
+    # ------------------------------------------------------------------ forward / backward
+    def _workspace(self, N: int, train: bool = True):
+        """Activations for N nodes: every round's (training) or two alternating
+        slots (inference)."""
+        torch = self.torch
+        ws = self._ws
+        if ws is not None and ws["N"] >= N and (ws["train"] or not train):
+            return ws
+        self._ws = None
+        bf = lambda *s: torch.zeros(s, dtype=torch.int16, device=self.dev)
+        f32 = lambda *s: torch.zeros(s, dtype=torch.float32, device=self.dev)
+        W = WIDTH
+        nr = ROUNDS if train else 1
+        ws = {"N": N, "train": train, "x0": bf(N, FEAT), "h1": bf(N, W),
+              "hin": [bf(N, 2 * W) for _ in range(nr + 1)],   # [h_r | agg_r]
+              "ab": [bf(N, 2 * W) for _ in range(nr)], "s": [bf(N, W) for _ in range(nr)],
+              "u": [bf(N, W) for _ in range(nr)], "g": bf(N, W), "y": f32(N, 3), "loss": f32(1)}
+        if train:
+            ws.update({"dy": f32(N, 3), "dh": f32(N, W), "dhb": bf(N, W), "t1": bf(N, W), "t2": bf(N, 2 * W),
+                       "dab": bf(N, 2 * W), "ds": f32(N, W)})
+        groups = 256
+        lib = nnlib()
+        scr = max(lib.dmph_nn_dw_scratch(N, 256, 256, groups), lib.dmph_nn_edge_scratch(N),
+                  lib.dmph_nn_head_scratch(N), lib.dmph_nn_mse_scratch(N))
+        ws["scratch"] = torch.zeros(scr // 4 + 1, dtype=torch.float32, device=self.dev)
+        ws["groups"] = groups
+        self._ws = ws
+        return ws
+
+    def features(self, vel_hist, cls, pos, train: bool = True):
+        """x0 [N, 32] bf16 from vel_hist [5, N, 3] (newest first), class ids, positions."""
+        N = pos.shape[0]
+        ws = self._workspace(N, train)
+        ft = NnFeat()
+        ft.vscale = self.vscale
+        for k in range(3):
+            ft.tlo[k] = float(self.tank_lo[k])
+            ft.thi[k] = float(self.tank_hi[k])
+        ft.inv_r = 1.0 / self.radius
+        _chk(nnlib().dmph_nn_features(_p(vel_hist), N, _p(cls), _p(pos), C.byref(ft), _p(ws["x0"]), self._s()),
+             "features")
+        return ws["x0"]
+
+    def _s(self) -> C.c_void_p:
+        return C.c_void_p(self.torch.cuda.current_stream(self.dev).cuda_stream)
+
+    def _gemm(self, X, ldx, layer: Layer, Y, ldy, M, relu, R=None, ldr=0, transposed=False, K=None, Nout=None):
+        w = layer.wbt if transposed else layer.wb
+        k = K if K is not None else (layer.n_out if transposed else layer.n_in)
+        n = Nout if Nout is not None else (layer.n_in if transposed else layer.n_out)
+        bias = None if transposed else layer.b
+        _chk(nnlib().dmph_nn_gemm(_p(X), ldx, _p(w), _p(bias), _p(R), ldr, _p(Y), ldy, M, k, n, 1 if relu else 0,
+                                  self.kstep, self._s()), "gemm")
+
+    def _dw(self, dY, lddy, X, ldx, layer: Layer, M):
+        ws = self._ws
+        _chk(nnlib().dmph_nn_gemm_dw(_p(dY), lddy, _p(X), ldx, _p(layer.gw), _p(layer.gb), M, layer.n_in, layer.n_out,
+                                     ws["groups"], _p(ws["scratch"]), 0, self._s()), "gemm_dw")
+
+    def forward(self, x0, pos, rp, ci, N: int, train: bool = True):
+        """Accelerations (scaled) y [N, 3] FP32; with `train` every round's
+        activations are kept for backward, else two slots alternate."""
+        lib = nnlib()
+        ws = self._workspace(N, train)
+        W = WIDTH
+        s = self._s()
+        slot = (lambda r: r) if train else (lambda r: 0)
+        hslot = (lambda r: r) if train else (lambda r: r % 2)
+        self._gemm(x0, FEAT, self.enc1, ws["h1"], W, N, True)
+        self._gemm(ws["h1"], W, self.enc2, ws["hin"][0], 2 * W, N, False)
+        for r, L in enumerate(self.rounds):
+            hin = ws["hin"][hslot(r)]
+            ab, sr, u = ws["ab"][slot(r)], ws["s"][slot(r)], ws["u"][slot(r)]
+            self._gemm(hin, 2 * W, L["ab"], ab, 2 * W, N, False)
+            _chk(lib.dmph_nn_edge_fwd(_p(ab), _p(pos), _p(rp), _p(ci), _p(L["c"].w), _p(L["c"].b),
+                                      _p(sr), N, C.c_float(self.radius), s), "edge_fwd")
+            # agg into the right half of hin_r
+            self._gemm(sr, W, L["w2"], hin[:, W:], 2 * W, N, False)
+            self._gemm(hin, 2 * W, L["n1"], u, W, N, True)
+            self._gemm(u, W, L["n2"], ws["hin"][hslot(r + 1)], 2 * W, N, False, R=hin, ldr=2 * W)
+        self._gemm(ws["hin"][hslot(ROUNDS)], 2 * W, self.dec1, ws["g"], W, N, True)
+        _chk(lib.dmph_nn_head(_p(ws["g"]), _p(self.dec2.w), _p(self.dec2.b), _p(ws["y"]), N, s), "head")
+        return ws["y"][:N]
+
+    def loss_and_backward(self, x0, pos, rp, ci, N: int, target, mask, count: int, clip: float = 1e30) -> float:
+        """MSE of forward() against target [N, 3] (clipped to +-clip) on mask;
+        fills every layer's gradient."""
+        lib = nnlib()
+        ws = self._ws
+        W = WIDTH
+        s = self._s()
+        _chk(lib.dmph_nn_mse(_p(ws["y"]), _p(target), _p(mask), N, count, C.c_float(clip), _p(ws["dy"]),
+                             _p(ws["loss"]), _p(ws["scratch"]), s), "mse")
+        # decoder
+        _chk(lib.dmph_nn_head_bwd(_p(ws["g"]), _p(self.dec2.w), _p(ws["dy"]), _p(ws["t1"]), _p(self.dec2.gw),
+                                  _p(self.dec2.gb), N, _p(ws["scratch"]), s), "head_bwd")
+        h = ws["hin"][ROUNDS]
+        self._dw(ws["t1"], W, h, 2 * W, self.dec1, N)
+        self._gemm(ws["t1"], W, self.dec1, ws["dhb"], W, N, False, transposed=True)
+        _chk(lib.dmph_nn_bf2f(_p(ws["dhb"]), W, _p(ws["dh"]), W, N, W, 0, s), "bf2f")
+        for r in reversed(range(ROUNDS)):
+            L = self.rounds[r]
+            hin = ws["hin"][r]
+            # h_{r+1} = h_r + n2(u_r): dn2 = dh
+            _chk(lib.dmph_nn_f2bf(_p(ws["dh"]), W, _p(ws["dhb"]), W, N, W, s), "f2bf")
+            self._dw(ws["dhb"], W, ws["u"][r], W, L["n2"], N)
+            self._gemm(ws["dhb"], W, L["n2"], ws["t1"], W, N, False, transposed=True)
+            _chk(lib.dmph_nn_relu_bwd(_p(ws["u"][r]), W, _p(ws["t1"]), W, _p(ws["t1"]), W, N, W, s), "relu_bwd")
+            self._dw(ws["t1"], W, hin, 2 * W, L["n1"], N)
+            self._gemm(ws["t1"], W, L["n1"], ws["t2"], 2 * W, N, False, transposed=True)   # [dh | dagg]
+            _chk(lib.dmph_nn_bf2f(_p(ws["t2"]), 2 * W, _p(ws["dh"]), W, N, W, 1, s), "bf2f")
+            dagg = ws["t2"][:, W:]
+            self._dw(dagg, 2 * W, ws["s"][r], W, L["w2"], N)
+            self._gemm(dagg, 2 * W, L["w2"], ws["t1"], W, N, False, transposed=True)        # dS (bf16)
+            _chk(lib.dmph_nn_bf2f(_p(ws["t1"]), W, _p(ws["ds"]), W, N, W, 0, s), "bf2f")
+            _chk(lib.dmph_nn_edge_bwd(_p(ws["ab"][r]), _p(pos), _p(rp), _p(ci), _p(L["c"].w), _p(L["c"].b),
+                                      _p(ws["ds"]), _p(ws["dab"]), _p(L["c"].gw), _p(L["c"].gb), N,
+                                      C.c_float(self.radius), _p(ws["scratch"]), 0, s), "edge_bwd")
+            self._dw(ws["dab"], 2 * W, hin, 2 * W, L["ab"], N)
+            self._gemm(ws["dab"], 2 * W, L["ab"], ws["dhb"], W, N, False, transposed=True)
+            _chk(lib.dmph_nn_bf2f(_p(ws["dhb"]), W, _p(ws["dh"]), W, N, W, 1, s), "bf2f")
+        # encoder: h0 = enc2(h1)
+        _chk(lib.dmph_nn_f2bf(_p(ws["dh"]), W, _p(ws["dhb"]), W, N, W, s), "f2bf")
+        self._dw(ws["dhb"], W, ws["h1"], W, self.enc2, N)
+        self._gemm(ws["dhb"], W, self.enc2, ws["t1"], W, N, False, transposed=True)
+        _chk(lib.dmph_nn_relu_bwd(_p(ws["h1"]), W, _p(ws["t1"]), W, _p(ws["t1"]), W, N, W, s), "relu_bwd")
+        self._dw(ws["t1"], W, x0, FEAT, self.enc1, N)
+        return float(ws["loss"].item())
+
+    def adam(self, lr: float, b1: float = 0.9, b2: float = 0.999, eps: float = 1e-8, wd: float = 0.0) -> None:
+        self.adam_step += 1
+        lib = nnlib()
+        s = self._s()
+        for l in self.layers:
+            _chk(lib.dmph_nn_adam(_p(l.w), _p(l.gw), _p(l.mw), _p(l.vw), l.w.numel(), C.c_float(lr), C.c_float(b1),
+                                  C.c_float(b2), C.c_float(eps), self.adam_step, C.c_float(wd), _p(l.wb), _p(l.wbt),
+                                  l.n_out, l.n_in, s), "adam")
+            if l.b is not None:
+                _chk(lib.dmph_nn_adam(_p(l.b), _p(l.gb), _p(l.mb), _p(l.vb), l.b.numel(), C.c_float(lr),
+                                      C.c_float(b1), C.c_float(b2), C.c_float(eps), self.adam_step, C.c_float(0.0),
+                                      None, None, 1, l.b.numel(), s), "adam")
+
+    def rollout_step(self, pos, vel_hist, y, cls, body, nbodies: int, h: float, body_sums) -> None:
+        """x, v history advanced by stride h from the network's output y."""
+        _chk(nnlib().dmph_nn_rollout_step(_p(pos), _p(vel_hist), _p(y), C.c_float(self.ascale), _p(cls), _p(body),
+                                          _p(body_sums), nbodies, C.c_float(h), pos.shape[0], self._s()), "rollout")
+
+
+# ---------------------------------------------------------------------------------------------
+# trajectories of the plain solver, training and rollouts
+# ---------------------------------------------------------------------------------------------
+
+STRIDE = 10   # solver steps per surrogate step
+
+
+class Trajectories:
+    """Frames of a batch of scenes from the plain FP32 solver (rigid debris) at
+    the given solver steps: positions and velocities of every particle, original
+    order, FP32, exported on the GPU (dmph_export_state); class (0 fluid, 1
+    debris, 2 wall), global body ids and the scene of each particle.
+    `samples` lists the frames usable as a training sample (the 4 frames before
+    it and the one after it are `stride` steps apart)."""
+
+    def __init__(self, scenes, params, steps: List[int], stride: int = STRIDE, device: str = "cuda:0"):
+        import torch
+        import dem_mph_amd as D
+        self.torch = torch
+        dev = torch.device(device)
+        t0 = _now()
+        s = D.Solver(D.Params.from_dict(params), scenes, precision=D.PREC_FP32, rigid=True)
+        s.autotune(3)
+        self.off = np.asarray(s.scene_offsets)
+        self.dt = params["Dt"]
+        self.stride = stride
+        self.nscene = len(scenes)
+        typ = np.concatenate([sc.type for sc in scenes])
+        self.cls_np = np.where(typ < 2, 0, np.where(typ < 4, 1, 2)).astype(np.int32)
+        body, boff = [], 0
+        for sc in scenes:
+            b = sc.body if sc.body is not None else np.full(sc.n, -1, np.int32)
+            body.append(np.where(b >= 0, b + boff, -1).astype(np.int32))
+            boff += int(b.max()) + 1 if (b >= 0).any() else 0
+        self.body_np = np.concatenate(body)
+        self.nbodies = boff
+        self.scene_np = np.repeat(np.arange(len(scenes), dtype=np.int32), np.diff(self.off))
+        self.steps = list(steps)
+        nf = len(self.steps)
+        self.pos = torch.empty((nf, s.n, 3), dtype=torch.float32, device=dev)
+        self.vel = torch.empty((nf, s.n, 3), dtype=torch.float32, device=dev)
+        cur = 0
+        for f, st in enumerate(self.steps):
+            if st > cur:
+                s.step(st - cur)
+                cur = st
+            s.export_state(self.pos[f].data_ptr(), self.vel[f].data_ptr())
+        self.sim_seconds = _now() - t0
+        self.solver_steps = cur
+        s.close()
+        self.cls = torch.tensor(self.cls_np, device=dev)
+        self.body = torch.tensor(self.body_np, device=dev)
+        self.scene = torch.tensor(self.scene_np, device=dev)
+        self.nframes = nf
+        st = self.steps
+        self.samples = [f for f in range(HIST - 1, nf - 1)
+                        if all(st[f - t] - st[f - t - 1] == stride for t in range(HIST - 1)) and st[f + 1] - st[f] == stride]
+
+    def centroids(self, pos_np: np.ndarray) -> np.ndarray:
+        """Debris-body centroids [nbodies, 3] of positions in original order."""
+        m = self.body_np >= 0
+        c = np.zeros((self.nbodies, 3))
+        np.add.at(c, self.body_np[m], pos_np[m])
+        return c / np.bincount(self.body_np[m], minlength=self.nbodies)[:, None]
+
+
+def _now() -> float:
+    import time
+    return time.perf_counter()
+
+
+def scales(traj: Trajectories) -> Tuple[float, float]:
+    """Feature and target scales from the data: 1 / std of moving particles'
+    velocities and of their 10 dt accelerations."""
+    torch = traj.torch
+    mv = traj.cls < 2
+    v = traj.vel[:, mv]
+    a = (traj.vel[1:, mv] - traj.vel[:-1, mv]) / (traj.stride * traj.dt)
+    return 1.0 / max(float(v.std()), 1e-6), 1.0 / max(float(a.std()), 1e-6)
+
+
+def _batch(traj: Trajectories, scenes: List[int], frame: int, ascale: float):
+    """Inputs of `scenes` (indices into traj) at `frame`: positions, velocity
+    history, class, body, scene ids (0..len-1), target, mask."""
+    torch = traj.torch
+    idx = torch.cat([torch.arange(int(traj.off[k]), int(traj.off[k + 1]), device=traj.pos.device) for k in scenes])
+    pos = traj.pos[frame, idx].contiguous()
+    vh = torch.stack([traj.vel[frame - t, idx] for t in range(HIST)]).contiguous()
+    cls = traj.cls[idx].contiguous()
+    sc = torch.cat([torch.full((int(traj.off[k + 1] - traj.off[k]),), q, dtype=torch.int32, device=pos.device)
+                    for q, k in enumerate(scenes)])
+    tgt = ((traj.vel[frame + 1, idx] - traj.vel[frame, idx]) * (ascale / (traj.stride * traj.dt))).contiguous()
+    mask = (cls < 2).to(torch.int32)
+    return pos, vh, cls, sc, tgt, mask, idx
+
+
+def window_steps(start: int, gap: int, windows: int, stride: int = STRIDE) -> List[int]:
+    """Solver steps of `windows` training windows of HIST + 1 frames each, `gap` steps apart."""
+    return [start + w * gap + k * stride for w in range(windows) for k in range(HIST + 1)]
+
+
+def train(sur: Surrogate, traj: Trajectories, iters: int, batch: int = 4, lr: float = 1e-3, seed: int = 0,
+          log=None) -> List[float]:
+    """Adam on random (scene batch, sample frame) pairs; cosine learning-rate decay."""
+    rng = np.random.default_rng(seed)
+    losses = []
+    for it in range(iters):
+        sc = sorted(rng.choice(traj.nscene, size=min(batch, traj.nscene), replace=False).tolist())
+        fr = int(traj.samples[int(rng.integers(len(traj.samples)))])
+        pos, vh, cls, scn, tgt, mask, _ = _batch(traj, sc, fr, sur.ascale)
+        n = pos.shape[0]
+        rp, ci, _ = sur.graph(pos, scn, len(sc))
+        x0 = sur.features(vh, cls, pos)
+        sur.forward(x0, pos, rp, ci, n)
+        # targets in units of their standard deviation, clipped at 5 (DEM contact impulses are heavy-tailed)
+        losses.append(sur.loss_and_backward(x0, pos, rp, ci, n, tgt, mask, int(mask.sum().item()), clip=5.0))
+        sur.adam(lr * 0.5 * (1.0 + np.cos(np.pi * it / max(iters, 1))))
+        if log and (it % 50 == 0 or it == iters - 1):
+            log("iter %d loss %.4f" % (it, losses[-1]))
+    return losses
+
+
+def rollout(sur: Optional[Surrogate], traj: Trajectories, start: int, steps: int, reanchor: int, tol: float,
+            baseline: Optional[str] = None):
+    """Roll every scene of `traj` forward `steps` surrogate steps from `start`
+    (or with a baseline without the network: "still" a = 0, "gravity" a = g).
+    Every `reanchor` steps the state is reset to the plain solver's frame; a
+    scene whose debris-centroid RMS error at a check exceeds `tol` falls back to
+    the plain solver for the rest.  Returns per-scene errors and timings."""
+    torch = traj.torch
+    dev = traj.pos.device
+    pos = traj.pos[start].clone()
+    vh = torch.stack([traj.vel[start - t] for t in range(HIST)]).contiguous()
+    n = pos.shape[0]
+    body_sums = torch.zeros((max(traj.nbodies, 1), 4), dtype=torch.float32, device=dev)
+    fell = np.zeros(traj.nscene, bool)
+    err_debris = np.zeros(traj.nscene)
+    err_fluid = np.zeros(traj.nscene)
+    h = traj.stride * traj.dt
+    g = np.array([0.0, 0.0, -9.81])
+    t_net = 0.0
+    for k in range(1, steps + 1):
+        t0 = _now()
+        if baseline is None:
+            rp, ci, _ = sur.graph(pos, traj.scene, traj.nscene)
+            x0 = sur.features(vh, traj.cls, pos, train=False)
+            y = sur.forward(x0, pos, rp, ci, n, train=False)
+            ascale = sur.ascale
+        else:
+            a = np.zeros(3) if baseline == "still" else g
+            y = torch.tensor(np.tile(a, (n, 1)).astype(np.float32), device=dev)
+            ascale = 1.0
+        _chk(nnlib().dmph_nn_rollout_step(_p(pos), _p(vh), _p(y), C.c_float(ascale), _p(traj.cls), _p(traj.body),
+                                          _p(body_sums), traj.nbodies, C.c_float(h), n,
+                                          C.c_void_p(torch.cuda.current_stream(dev).cuda_stream)), "rollout")
+        torch.cuda.synchronize(dev)
+        t_net += _now() - t0
+        if k % reanchor == 0 or k == steps:
+            ref = traj.pos[start + k].cpu().numpy().astype(np.float64)
+            cur = pos.cpu().numpy().astype(np.float64)
+            cr, cc = traj.centroids(ref), traj.centroids(cur)
+            boff = 0
+            for q in range(traj.nscene):
+                sl = slice(int(traj.off[q]), int(traj.off[q + 1]))
+                bq = np.unique(traj.body_np[sl][traj.body_np[sl] >= 0])
+                e = float(np.sqrt(((cc[bq] - cr[bq]) ** 2).sum(axis=1).mean())) if len(bq) else 0.0
+                fl = traj.cls_np[sl] == 0
+                ef = float(np.sqrt(((cur[sl][fl] - ref[sl][fl]) ** 2).sum(axis=1).mean())) if fl.any() else 0.0
+                if not fell[q]:
+                    err_debris[q] = max(err_debris[q], e)
+                    err_fluid[q] = max(err_fluid[q], ef)
+                    if e > tol:
+                        fell[q] = True
+            if k < steps:   # re-anchor on the plain solver's state
+                pos.copy_(traj.pos[start + k])
+                vh.copy_(torch.stack([traj.vel[start + k - t] for t in range(HIST)]))
+    return {"debris_rms_max": err_debris, "fluid_rms_max": err_fluid, "fell_back": fell, "seconds": t_net,
+            "steps": steps}
+
+
+
+def save(sur: Surrogate, path: str, **meta) -> None:
+    sd = sur.state_dict()
+    np.savez(path, vscale=sur.vscale, ascale=sur.ascale, d=sur.d, tank_lo=sur.tank_lo, tank_hi=sur.tank_hi,
+             meta=np.array(repr(meta)), **{"p/" + k: v.astype(np.float32) for k, v in sd.items()})
+
+
+def load(path: str, kstep: int = 32) -> Surrogate:
+    z = np.load(path, allow_pickle=False)
+    sur = Surrogate(d=float(z["d"]), tank_lo=z["tank_lo"], tank_hi=z["tank_hi"], vscale=float(z["vscale"]),
+                    ascale=float(z["ascale"]), kstep=kstep)
+    sur.load_state({k[2:]: z[k].astype(np.float64) for k in z.files if k.startswith("p/")})
+    return sur
+
+
+def evaluate(sur: Surrogate, scenes, params, start: int, steps: int, reanchor: int, tol: float) -> Dict[str, object]:
+    """Held-out scenes: reference frames from the plain solver, then the
+    surrogate's rollout and the two baselines without a network."""
+    traj = Trajectories(scenes, params, [start - (HIST - 1) * STRIDE + k * STRIDE for k in range(steps + HIST)])
+    out = {"scenes": len(scenes), "particles": int(traj.off[-1]), "start_time_s": start * traj.dt,
+           "horizon_s": steps * STRIDE * traj.dt, "reanchor_steps": reanchor, "tolerance_m": tol}
+    for name, base in (("surrogate", None), ("still", "still"), ("gravity", "gravity")):
+        r = rollout(sur if base is None else None, traj, HIST - 1, steps, reanchor, tol, baseline=base)
+        out[name] = {"debris_centroid_rms_m_mean": float(r["debris_rms_max"].mean()),
+                     "debris_centroid_rms_m_max": float(r["debris_rms_max"].max()),
+                     "fluid_rms_m_mean": float(r["fluid_rms_max"].mean()),
+                     "fell_back": int(r["fell_back"].sum()), "seconds": r["seconds"]}
+    return out
+
+
+def main(argv=None) -> int:
+    import argparse
+    import json
+    import sys
+    from . import scenes as S
+    ap = argparse.ArgumentParser(description="train and evaluate the surrogate on C3-style scenes")
+    ap.add_argument("--train-scenes", type=int, default=128)
+    ap.add_argument("--eval-scenes", type=int, default=32)
+    ap.add_argument("--seed", type=int, default=11)
+    ap.add_argument("--windows", type=int, default=16)
+    ap.add_argument("--gap", type=int, default=300, help="solver steps between training windows")
+    ap.add_argument("--iters", type=int, default=1500)
+    ap.add_argument("--batch", type=int, default=4)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--eval-start", type=int, default=2000, help="solver step where rollouts start")
+    ap.add_argument("--eval-steps", type=int, default=100, help="surrogate steps (10 solver steps each)")
+    ap.add_argument("--reanchor", type=int, default=200)
+    ap.add_argument("--tol", type=float, default=0.008, help="debris-centroid RMS tolerance (1%% of the tank length)")
+    ap.add_argument("--out", default=os.path.join(PKG, "surrogate_c3.npz"))
+    ap.add_argument("--eval-only", action="store_true")
+    a = ap.parse_args(argv)
+    log = lambda m: print("surrogate: " + m, file=sys.stderr, flush=True)
+    params = S.params_c3()
+    rep = {}
+    if not a.eval_only:
+        sc = S.ensemble_c3(a.train_scenes, seed=a.seed)
+        log("training data: %d scenes, %d windows" % (len(sc), a.windows))
+        traj = Trajectories(sc, params, window_steps(HIST * STRIDE, a.gap, a.windows))
+        vs, asc = scales(traj)
+        log("data %.1f s; vscale %.3g ascale %.3g" % (traj.sim_seconds, vs, asc))
+        sur = Surrogate(d=0.01, tank_lo=[0, 0, 0], tank_hi=[0.8, 0.4, 0.6], vscale=vs, ascale=asc, seed=a.seed)
+        t0 = _now()
+        losses = train(sur, traj, a.iters, a.batch, a.lr, a.seed, log)
+        rep["train"] = {"scenes": len(sc), "windows": a.windows, "iters": a.iters, "batch": a.batch,
+                        "seconds": _now() - t0, "loss_first": losses[0], "loss_last10": float(np.mean(losses[-10:]))}
+        save(sur, a.out, **rep["train"])
+        del traj
+    else:
+        sur = load(a.out)
+    ev = S.ensemble_c3(a.eval_scenes, seed=a.seed + 1000)   # held-out seeds
+    rep["eval"] = evaluate(sur, ev, params, a.eval_start, a.eval_steps, a.reanchor, a.tol)
+    print(json.dumps(rep))
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())

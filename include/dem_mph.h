@@ -160,6 +160,13 @@ int dmph_get_field(dmph_handle *h, int field, int order, void *dst);
  * next DEM sub-step.  Synchronous. */
 int dmph_set_field(dmph_handle *h, int field, int order, const void *src);
 
+/* Positions (absolute, wrapped into the domain) and velocities as FP32 [n][3]
+ * in original order, written to DEVICE buffers on the handle's device (either
+ * may be NULL): a copy that stays on the GPU, for coupling with other device
+ * code (the surrogate records trajectories this way).  No reference
+ * counterpart.  Synchronous. */
+int dmph_export_state(dmph_handle *h, float *pos, float *vel);
+
 /* Rigid-body state (no reference counterpart): per body X[3], V[3], q[4] (w,x,y,z), L[3], mass. */
 int dmph_get_bodies(dmph_handle *h, double *x, double *v, double *q, double *l, double *mass);
 

@@ -1,0 +1,103 @@
+/* dem_mph_nn.h — C ABI of the message-passing surrogate kernels (libdemmph_nn.so).
+ *
+ * No reference counterpart: the reference (source/DEM-MPH.cpp) has no learned
+ * component.  These are the building blocks of a graph network that predicts
+ * particle accelerations at a 10 dt stride, trained on trajectories of the
+ * plain solver; dem_mph_amd/surrogate.py drives them.  All pointers are device
+ * pointers; `stream` is a hipStream_t.  bf16 values are raw 16-bit storage.
+ * Weights of a layer with K inputs and N outputs are [N][K].  Calls return 0,
+ * -1 for unsupported shapes, -2 when a capacity is too small, -5 on a HIP error.
+ */
+#ifndef DEM_MPH_NN_H
+#define DEM_MPH_NN_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* uniform grid for the radius graph: cell width >= radius; per scene cells */
+typedef struct NnGrid {
+    float lo[3];
+    float inv_w;
+    int cc[3];
+    int ncell;    /* cc[0] * cc[1] * cc[2] */
+    int nscene;   /* scenes in the batch (node scene ids < nscene) */
+} NnGrid;
+
+/* node features: velocity scale, tank interior box, 1 / radius */
+typedef struct NnFeat {
+    float vscale;
+    float tlo[3], thi[3];
+    float inv_r;
+} NnFeat;
+
+/* Y[:, 0:N) (row stride ldy) = act(X[M][K] W^T + bias [+ R]); K in {32, 64, 128, 256},
+ * N % 128 == 0; relu 0/1; kstep 16 (v_mfma_f32_16x16x16_bf16) or 32 (gfx950 16x16x32) */
+int dmph_nn_gemm(const uint16_t *X, int ldx, const uint16_t *W, const float *bias, const uint16_t *R, int ldr,
+                 uint16_t *Y, int ldy, int M, int K, int N, int relu, int kstep, void *stream);
+
+/* dW[N][K] (+)= dY^T X, db[N] (+)= sum_m dY (db may be NULL); K in {32, 128, 256},
+ * N % 128 == 0; `groups` row chunks summed in a fixed order (deterministic);
+ * scratch of dmph_nn_dw_scratch bytes */
+size_t dmph_nn_dw_scratch(int M, int K, int N, int groups);
+int dmph_nn_gemm_dw(const uint16_t *dY, int lddy, const uint16_t *X, int ldx, float *dW, float *db, int M, int K, int N,
+                    int groups, float *scratch, int accumulate, void *stream);
+
+/* S_i = sum_{j in row i} relu(A_i + B_j + Wc eps_ij + bc); AB = [A | B] as [N][256],
+ * eps_ij = ((x_j - x_i) / radius, |x_j - x_i| / radius), Wc [128][4], bc [128], S [N][128] */
+int dmph_nn_edge_fwd(const uint16_t *AB, const float *pos, const int *rp, const int *ci, const float *Wc,
+                     const float *bc, uint16_t *S, int N, float radius, void *stream);
+/* gradients of edge_fwd given dS [N][128] FP32: dAB [N][256] bf16, dWc, dbc (+)=;
+ * the graph must be symmetric; scratch of dmph_nn_edge_scratch bytes */
+size_t dmph_nn_edge_scratch(int N);
+int dmph_nn_edge_bwd(const uint16_t *AB, const float *pos, const int *rp, const int *ci, const float *Wc,
+                     const float *bc, const float *dS, uint16_t *dAB, float *dWc, float *dbc, int N, float radius,
+                     float *scratch, int accumulate, void *stream);
+
+/* radius graph (CSR rp[N + 1], ci[cap]) of the nodes at pos [N][3] (scene ids or NULL):
+ * rows ascending by cell then node; *nnz_out = edges; returns -2 if cap < nnz (call again).
+ * Work arrays: key, order, sorted, deg [N]; counts, start, fill [cells * scenes + 1]. */
+int dmph_nn_graph(const float *pos, const int *scene, int N, const NnGrid *grid, float radius, int *key, int *counts,
+                  int *start, int *fill, int *order, int *sorted, int *deg, int *rp, int *ci, int cap, int *nnz_out,
+                  void *stream);
+
+/* node features X [N][32] bf16 from vel_hist [5][N][3] (newest first), class 0/1/2, positions */
+int dmph_nn_features(const float *vel_hist, int N, const int *cls, const float *pos, const NnFeat *ft, uint16_t *X,
+                     void *stream);
+
+/* output head: Y [N][3] = H Wd^T + bd (Wd [3][128]); backward with the ReLU mask of H */
+int dmph_nn_head(const uint16_t *H, const float *Wd, const float *bd, float *Y, int N, void *stream);
+size_t dmph_nn_head_scratch(int N);
+int dmph_nn_head_bwd(const uint16_t *H, const float *Wd, const float *dY, uint16_t *dH, float *dWd, float *dbd, int N,
+                     float *scratch, void *stream);
+
+/* mean squared error over masked nodes (count of them) against targets clipped to
+ * [-clip, clip]: loss (device float), dY */
+size_t dmph_nn_mse_scratch(int N);
+int dmph_nn_mse(const float *Y, const float *T, const int *mask, int N, int count, float clip, float *dY, float *loss,
+                float *scratch, void *stream);
+
+int dmph_nn_relu_bwd(const uint16_t *Y, int ldy, const uint16_t *dY, int lddy, uint16_t *dX, int lddx, int M, int n,
+                     void *stream);
+int dmph_nn_f2bf(const float *x, int ldx, uint16_t *y, int ldy, int M, int n, void *stream);
+int dmph_nn_bf2f(const uint16_t *x, int ldx, float *y, int ldy, int M, int n, int accumulate, void *stream);
+
+/* One rollout step of stride h from predicted (scaled) accelerations Y [N][3]:
+ * fluid (class 0) v += h Y / ascale, debris (class 1) the mean over its body
+ * (body ids < nbodies, sums in body_sums [nbodies][4]), walls (class 2) fixed;
+ * x += h v; the velocity history [5][N][3] shifts by one (newest first). */
+int dmph_nn_rollout_step(float *pos, float *vel_hist, const float *Y, float ascale, const int *cls, const int *body,
+                         float *body_sums, int nbodies, float h, int N, void *stream);
+
+/* Adam step `step` (1-based) on FP32 master weights w[rows][cols]; refreshes the
+ * bf16 copy wb [rows][cols] and transpose wbt [cols][rows] when not NULL */
+int dmph_nn_adam(float *w, const float *g, float *m, float *v, int n, float lr, float b1, float b2, float eps, int step,
+                 float wd, uint16_t *wb, uint16_t *wbt, int rows, int cols, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

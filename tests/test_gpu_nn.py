@@ -1,0 +1,164 @@
+"""Surrogate kernels on the GPU (libdemmph_nn.so through its C ABI) against
+numpy and the FP64 network oracle (oracle/nn_oracle.py)."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import nn_oracle as NO
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def _bf16_bits(a: np.ndarray) -> np.ndarray:
+    """round-to-nearest-even bf16 bits of float32 values"""
+    u = np.ascontiguousarray(a, np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16
+    return u.astype(np.uint16)
+
+
+def _bf16_float(bits: np.ndarray) -> np.ndarray:
+    return (bits.astype(np.uint32) << 16).view(np.float32).astype(np.float64)
+
+
+def _dev_bf16(a):
+    return torch.tensor(_bf16_bits(a).view(np.int16), device="cuda:0")
+
+
+def _host_bf16(t):
+    return _bf16_float(t.cpu().numpy().view(np.uint16))
+
+
+def _p(t):
+    return C.c_void_p(t.data_ptr()) if t is not None else C.c_void_p(0)
+
+
+@pytest.fixture(scope="module")
+def S():
+    from dem_mph_amd import surrogate as S
+    return S
+
+
+@pytest.mark.parametrize("kstep", [16, 32])
+@pytest.mark.parametrize("K,N,relu,res", [(32, 128, 1, 0), (128, 128, 0, 1), (256, 128, 1, 0), (128, 256, 0, 0)])
+def test_gemm_matches_numpy(S, kstep, K, N, relu, res):
+    rng = np.random.default_rng(K + N + kstep)
+    M = 300
+    X = rng.standard_normal((M, K)).astype(np.float32)
+    Wt = (rng.standard_normal((N, K)) / np.sqrt(K)).astype(np.float32)
+    b = rng.standard_normal(N).astype(np.float32)
+    R = rng.standard_normal((M, N)).astype(np.float32)
+    Xd, Wd, Rd = _dev_bf16(X), _dev_bf16(Wt), _dev_bf16(R)
+    bd = torch.tensor(b, device="cuda:0")
+    Y = torch.zeros((M, N), dtype=torch.int16, device="cuda:0")
+    s = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    rc = S.nnlib().dmph_nn_gemm(_p(Xd), K, _p(Wd), _p(bd), _p(Rd) if res else None, N, _p(Y), N, M, K, N, relu,
+                                kstep, s)
+    assert rc == 0
+    ref = _host_bf16(Xd) @ _host_bf16(Wd).T + b + (_host_bf16(Rd) if res else 0.0)
+    if relu:
+        ref = np.maximum(ref, 0.0)
+    got = _host_bf16(Y)
+    err = np.abs(got - ref)
+    assert err.max() <= 1e-2 * np.abs(ref).max(), err.max()
+    assert np.abs(got - ref).mean() <= 2e-3 * np.abs(ref).mean() + 1e-6
+
+
+@pytest.mark.parametrize("K,N", [(32, 128), (128, 128), (256, 128), (128, 256)])
+def test_gemm_dw_matches_numpy(S, K, N):
+    rng = np.random.default_rng(K * N)
+    M = 1000
+    dY = rng.standard_normal((M, N)).astype(np.float32)
+    X = rng.standard_normal((M, K)).astype(np.float32)
+    dYd, Xd = _dev_bf16(dY), _dev_bf16(X)
+    dW = torch.zeros((N, K), dtype=torch.float32, device="cuda:0")
+    db = torch.zeros(N, dtype=torch.float32, device="cuda:0")
+    lib = S.nnlib()
+    scr = torch.zeros(lib.dmph_nn_dw_scratch(M, K, N, 7) // 4 + 1, dtype=torch.float32, device="cuda:0")
+    s = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    assert lib.dmph_nn_gemm_dw(_p(dYd), N, _p(Xd), K, _p(dW), _p(db), M, K, N, 7, _p(scr), 0, s) == 0
+    ref = _host_bf16(dYd).T @ _host_bf16(Xd)
+    np.testing.assert_allclose(dW.cpu().numpy(), ref, rtol=1e-4, atol=1e-3)
+    np.testing.assert_allclose(db.cpu().numpy(), _host_bf16(dYd).sum(axis=0), rtol=1e-4, atol=1e-3)
+
+
+def test_radius_graph_matches_brute_force(S):
+    rng = np.random.default_rng(5)
+    pos = rng.uniform(0.0, 0.3, (3000, 3)).astype(np.float32)
+    sur = S.Surrogate(d=0.01, tank_lo=[0, 0, 0], tank_hi=[0.3, 0.3, 0.3], vscale=1.0, ascale=1.0)
+    rp, ci, nnz = sur.graph(torch.tensor(pos, device="cuda:0"))
+    rp, ci = rp.cpu().numpy(), ci.cpu().numpy()[:nnz]
+    brp, bci = NO.radius_graph(pos.astype(np.float64), sur.radius)
+    # the GPU tests r^2 in FP32: pairs within a rounding of the radius may differ
+    d = np.linalg.norm(pos[:, None] - pos[None], axis=-1)
+    for i in range(len(pos)):
+        a = set(ci[rp[i]:rp[i + 1]].tolist())
+        b = set(bci[brp[i]:brp[i + 1]].tolist())
+        for j in a ^ b:
+            assert abs(d[i, j] - sur.radius) < 1e-6
+        assert len(ci[rp[i]:rp[i + 1]]) == len(a)   # no duplicates
+    assert nnz > 10 * len(pos)
+
+
+def _small_problem(S, seed=0):
+    from dem_mph_amd import scenes as SC
+    sc = SC.ensemble_c3(1, seed=seed, d=0.02, interior=(0.4, 0.2, 0.3), fluid_range=(200.0, 600.0))[0]
+    sur = S.Surrogate(d=sc.d, tank_lo=[0, 0, 0], tank_hi=[0.4, 0.2, 0.3], vscale=3.0, ascale=0.1, seed=seed)
+    n = sc.n
+    rng = np.random.default_rng(seed)
+    pos = torch.tensor(sc.pos.astype(np.float32), device="cuda:0")
+    vh = torch.tensor(rng.standard_normal((5, n, 3)).astype(np.float32) * 0.2, device="cuda:0")
+    cls = torch.tensor(np.where(sc.type < 2, 0, np.where(sc.type < 4, 1, 2)).astype(np.int32), device="cuda:0")
+    tgt = torch.tensor(rng.standard_normal((n, 3)).astype(np.float32), device="cuda:0")
+    mask = (cls < 2).to(torch.int32)
+    return sur, sc, pos, vh, cls, tgt, mask
+
+
+def test_network_gradients_match_fp64_oracle(S):
+    """bf16 MFMA forward/backward against the FP64 oracle on the same parameters,
+    inputs and graph: outputs and every parameter's gradient within 5% (norm),
+    gradient directions within 0.99 cosine."""
+    sur, sc, pos, vh, cls, tgt, mask = _small_problem(S)
+    n = sc.n
+    rp, ci, nnz = sur.graph(pos)
+    x0 = sur.features(vh, cls, pos)
+    y = sur.forward(x0, pos, rp, ci, n).clone()
+    count = int(mask.sum().item())
+    sur.loss_and_backward(x0, pos, rp, ci, n, tgt, mask, count)
+    P = sur.state_dict()
+    X0 = _host_bf16(x0[:n])
+    yr, cache = NO.forward(P, X0, pos.cpu().numpy().astype(np.float64), rp.cpu().numpy(), ci.cpu().numpy()[:nnz],
+                           sur.radius)
+    m = mask.cpu().numpy().astype(float)
+    Gr = NO.backward(P, cache, yr, tgt.cpu().numpy().astype(np.float64), m, count)
+    ey = np.linalg.norm(y.cpu().numpy() - yr) / np.linalg.norm(yr)
+    assert ey < 0.05, ey
+    G = sur.grads()
+    worst = {}
+    for k in Gr:
+        a, b = G[k].ravel(), Gr[k].ravel()
+        rel = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
+        cos = a @ b / max(np.linalg.norm(a) * np.linalg.norm(b), 1e-30)
+        worst[k] = (rel, cos)
+    bad = {k: v for k, v in worst.items() if not (v[0] < 0.05 and v[1] > 0.99)}
+    assert not bad, bad
+
+
+def test_training_reduces_loss(S):
+    """Adam on a target the inputs determine (twice the newest velocity): the
+    loss falls at least tenfold in 150 steps."""
+    sur, sc, pos, vh, cls, tgt, mask = _small_problem(S, seed=1)
+    n = sc.n
+    tgt = (2.0 * vh[0]).contiguous()
+    rp, ci, nnz = sur.graph(pos)
+    x0 = sur.features(vh, cls, pos)
+    count = int(mask.sum().item())
+    losses = []
+    for it in range(150):
+        sur.forward(x0, pos, rp, ci, n)
+        losses.append(sur.loss_and_backward(x0, pos, rp, ci, n, tgt, mask, count))
+        sur.adam(1e-3)
+    assert np.isfinite(losses).all()
+    assert losses[-1] < 0.1 * losses[0], (losses[0], losses[-1])
