@@ -244,6 +244,43 @@ def ensemble_bench(D, nscenes: int, steps: int, warmup: int, seed: int, sample: 
                                          % (len(idx), nscenes, n_s)}
     out.update(res)
     out["batched_over_sequential"] = res["batched_mixed"]["sim_s_per_wall_s"] / res["sequential_plain"]["sim_s_per_wall_s"]
+    return out, scenes
+
+
+SURROGATE_WEIGHTS = os.path.join(ROOT, "dem_mph_amd", "surrogate_c3.npz")
+
+
+def surrogate_bench(D, scenes, seq_ms_per_dt: float, batched_ms_per_dt: float, chunk: int, steps: int,
+                    eval_scenes: int, eval_steps: int, seed: int):
+    """The learned surrogate on the ensemble: throughput of surrogate + fallback
+    (one plain-solver stride to re-anchor every 200 surrogate steps, charged at
+    the batched solver's rate) on all scenes in chunks, and accuracy on held-out
+    scenes against the plain solver next to the two baselines without a network."""
+    from dem_mph_amd import scenes as S
+    from dem_mph_amd import surrogate as SG
+    if not os.path.exists(SURROGATE_WEIGHTS):
+        return {"skipped": "no trained weights at dem_mph_amd/surrogate_c3.npz"}
+    sur = SG.load(SURROGATE_WEIGHTS)
+    params = S.params_c3()
+    dt = params["Dt"]
+    t_step = 0.0
+    finite = True
+    for c0 in range(0, len(scenes), chunk):
+        r = SG.advance_batch(sur, scenes[c0:c0 + chunk], steps, dt)
+        t_step += r["seconds_per_step"]
+        finite = finite and r["finite"]
+        print("bench: surrogate chunk %d: %.1f ms/step for %d nodes" % (c0 // chunk, r["seconds_per_step"] * 1e3,
+                                                                        r["nodes"]), file=sys.stderr, flush=True)
+    n = sum(sc.n for sc in scenes)
+    reanchor = 200
+    t_total = t_step + (SG.STRIDE * batched_ms_per_dt * 1e-3) / reanchor
+    out = {"surrogate_ms_per_step": t_step * 1e3, "stride_dt": SG.STRIDE,
+           "sim_s_per_wall_s": SG.STRIDE * dt / t_total,
+           "particle_steps_per_s": n * SG.STRIDE / t_total, "finite": finite,
+           "vs_sequential_plain": (SG.STRIDE * dt / t_total) / (dt / (seq_ms_per_dt * 1e-3)),
+           "vs_batched_plain": (SG.STRIDE * dt / t_total) / (dt / (batched_ms_per_dt * 1e-3))}
+    ev = S.ensemble_c3(eval_scenes, seed=seed + 1000)
+    out["accuracy"] = SG.evaluate(sur, ev, params, 2000, eval_steps, reanchor, 0.008)
     return out
 
 
@@ -261,6 +298,9 @@ def main(argv=None) -> int:
     ap.add_argument("--ensemble-scenes", type=int, default=512)
     ap.add_argument("--ensemble-steps", type=int, default=40)
     ap.add_argument("--ensemble-sample", type=int, default=32)
+    ap.add_argument("--no-surrogate", action="store_true", help="skip the learned-surrogate measurement")
+    ap.add_argument("--surrogate-chunk", type=int, default=128, help="scenes per surrogate batch")
+    ap.add_argument("--surrogate-eval-scenes", type=int, default=16)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="also run one device-to-device copy of MIB MiB (counter calibration)")
     a = ap.parse_args(argv)
@@ -336,7 +376,14 @@ def main(argv=None) -> int:
         out["cpu_baseline"] = None
     if not a.no_ensemble:
         print("bench: batched ensemble", file=sys.stderr, flush=True)
-        out["ensemble"] = ensemble_bench(D, a.ensemble_scenes, a.ensemble_steps, 5, a.seed, a.ensemble_sample)
+        ens, scenes = ensemble_bench(D, a.ensemble_scenes, a.ensemble_steps, 5, a.seed, a.ensemble_sample)
+        out["ensemble"] = ens
+        if not a.no_surrogate:
+            print("bench: learned surrogate", file=sys.stderr, flush=True)
+            ens["surrogate_plus_fallback"] = surrogate_bench(
+                D, scenes, ens["sequential_plain"]["ms_per_step"], ens["batched_mixed"]["ms_per_step"],
+                a.surrogate_chunk, 3, a.surrogate_eval_scenes, 100, a.seed)
+        del scenes
     print(json.dumps(out))
     return 0
 

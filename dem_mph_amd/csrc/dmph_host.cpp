@@ -895,9 +895,18 @@ class Engine final : public dmph_handle {
         if (!idx) return DMPH_OK;
         std::vector<uint32_t> list;
         TRY(download(list, a.nlist, (size_t)g.kcap * n));
+        std::vector<int> nn(n);
+        if (std::is_same<R, double>::value) {
+            for (int i = 0; i < n; i++) nn[i] = cnt[i];
+        } else {
+            TRY(download(nn, a.nnear, n));
+        }
+        // FP32 rows: near entries at the front, far entries at the back of the capacity
         for (int i = 0; i < n; i++)
-            for (int k = 0; k < cnt[i]; k++)
-                idx[pt[i] + k] = (int)(list[((size_t)(k >> 3) * n + i) * 8 + (k & 7)] >> 6);
+            for (int k = 0; k < cnt[i]; k++) {
+                const int q = k < nn[i] ? k : k + (g.kcap - cnt[i]);
+                idx[pt[i] + k] = (int)(list[((size_t)(q >> 3) * n + i) * 8 + (q & 7)] >> 6);
+            }
         return DMPH_OK;
     }
 

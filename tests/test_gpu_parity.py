@@ -326,3 +326,31 @@ def test_autotune_keeps_state_and_tolerance():
     acc = np.linalg.norm((F[fl] - Fr[fl]) / m[fl, None]) / np.linalg.norm(Fr[fl] / m[fl, None])
     assert acc <= 1e-4
 
+
+@pytest.mark.parametrize("name", ["mix3d", "mix2d"])
+def test_fp32_rows_hold_the_fp64_pair_sets(name):
+    """FP32 rows keep near entries at the front and far ones at the back of the
+    row's capacity; read through dmph_pairs they hold the same pairs as the FP64
+    rows (canonical order), up to pairs within a rounding of the cutoff."""
+    g = G.load(name)
+    s32 = _solver(g, precision=D.PREC_FP32, sort_mode=D.SORT_CANONICAL)
+    s64 = _solver(g, precision=D.PREC_FP64, sort_mode=D.SORT_CANONICAL)
+    np.testing.assert_array_equal(s32.field("id", D.ORDER_SLOT), s64.field("id", D.ORDER_SLOT))
+    p32, i32 = s32.pairs()
+    p64, i64 = s64.pairs()
+    x = s64.field("pos", D.ORDER_SLOT)
+    L = g["domain_max"] - g["domain_min"]
+    cut = s64.info().max_radius + 0.1 * float(g["spacing"])
+    differ = 0
+    for i in range(len(p64) - 1):
+        a = set(i32[p32[i]:p32[i + 1]].tolist())
+        b = set(i64[p64[i]:p64[i + 1]].tolist())
+        assert len(a) == p32[i + 1] - p32[i]   # no duplicates
+        for j in a ^ b:
+            dx = x[j] - x[i]
+            dx -= L * np.round(dx / L)
+            r = np.linalg.norm(dx)
+            assert abs(r - cut) < 1e-6 * cut, (i, j, r, cut)
+            differ += 1
+    assert differ <= 4
+

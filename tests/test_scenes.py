@@ -62,3 +62,17 @@ def test_ensemble_is_seeded():
     for x, y in zip(a, b):
         np.testing.assert_array_equal(x.pos, y.pos)
     assert any(x.n != z.n or not np.array_equal(x.pos, z.pos) for x, z in zip(a, c))
+
+
+def test_surrogate_training_windows():
+    """Training windows: HIST + 1 frames STRIDE steps apart; every window yields
+    exactly one sample frame (the one with 4 frames before and 1 after)."""
+    from dem_mph_amd import surrogate as SG
+    st = SG.window_steps(50, 300, 4)
+    assert len(st) == 4 * (SG.HIST + 1)
+    assert st[:6] == [50, 60, 70, 80, 90, 100] and st[6] == 350
+    assert SG.sample_frames(st) == [4, 10, 16, 22]
+    # a continuous record: every frame with 4 before and 1 after
+    assert SG.sample_frames(list(range(0, 100, 10))) == [4, 5, 6, 7, 8]
+    # a gap breaks the history: only frame 7 (75) has 4 evenly spaced frames before it and one after
+    assert SG.sample_frames([0, 10, 20, 35, 45, 55, 65, 75, 85]) == [7]
