@@ -459,8 +459,8 @@ __global__ void k_head(const bf16_t *H, const float *Wd, const float *bd, float 
     }
 }
 
-// loss = sum_{mask} |y - t|^2 / count; dY = 2 (y - t) / count on masked nodes (else 0);
-// per-block loss partials in Pl
+// loss = sum_i w_i |y_i - clip(t_i)|^2 / count with integer weights w_i = mask[i]
+// (0: not in the loss); dY = 2 w (y - t) / count; per-block loss partials in Pl
 __global__ void k_mse(const float *Yo, const float *T, const int *mask, int N, float inv_count, float clip, float *dY,
                       float *Pl) {
     __shared__ float red[256];
@@ -469,9 +469,10 @@ __global__ void k_mse(const float *Yo, const float *T, const int *mask, int N, f
     if (i < N) {
         for (int c = 0; c < 3; c++) {
             const float t = fminf(fmaxf(T[3 * i + c], -clip), clip);
-            const float d = (mask[i] ? Yo[3 * i + c] - t : 0.f);
-            l += d * d;
-            dY[3 * i + c] = 2.f * d * inv_count;
+            const float w = (float)mask[i];   // per-node weight (0 = not in the loss)
+            const float d = w > 0.f ? Yo[3 * i + c] - t : 0.f;
+            l += w * d * d;
+            dY[3 * i + c] = 2.f * w * d * inv_count;
         }
     }
     red[threadIdx.x] = l * inv_count;

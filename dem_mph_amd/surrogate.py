@@ -424,9 +424,7 @@ class Trajectories:
         self.body = torch.tensor(self.body_np, device=dev)
         self.scene = torch.tensor(self.scene_np, device=dev)
         self.nframes = nf
-        st = self.steps
-        self.samples = [f for f in range(HIST - 1, nf - 1)
-                        if all(st[f - t] - st[f - t - 1] == stride for t in range(HIST - 1)) and st[f + 1] - st[f] == stride]
+        self.samples = sample_frames(self.steps, stride)
 
     def centroids(self, pos_np: np.ndarray) -> np.ndarray:
         """Debris-body centroids [nbodies, 3] of positions in original order."""
@@ -434,6 +432,14 @@ class Trajectories:
         c = np.zeros((self.nbodies, 3))
         np.add.at(c, self.body_np[m], pos_np[m])
         return c / np.bincount(self.body_np[m], minlength=self.nbodies)[:, None]
+
+
+def sample_frames(steps: List[int], stride: int = STRIDE) -> List[int]:
+    """Frames usable as training samples: the HIST - 1 frames before and the
+    one after are `stride` solver steps apart."""
+    return [f for f in range(HIST - 1, len(steps) - 1)
+            if all(steps[f - t] - steps[f - t - 1] == stride for t in range(HIST - 1))
+            and steps[f + 1] - steps[f] == stride]
 
 
 def _now() -> float:
@@ -451,19 +457,29 @@ def scales(traj: Trajectories) -> Tuple[float, float]:
     return 1.0 / max(float(v.std()), 1e-6), 1.0 / max(float(a.std()), 1e-6)
 
 
-def _batch(traj: Trajectories, scenes: List[int], frame: int, ascale: float):
+DEBRIS_WEIGHT = 5   # loss weight of debris particles (the rollout metric is the debris centroids)
+
+
+def _batch(traj: Trajectories, scenes: List[int], frame: int, ascale: float, noise: float = 0.0, gen=None):
     """Inputs of `scenes` (indices into traj) at `frame`: positions, velocity
-    history, class, body, scene ids (0..len-1), target, mask."""
+    history, class, body, scene ids (0..len-1), target, per-node loss weight.
+    With `noise` (m/s), moving particles' history gets Gaussian noise and the
+    target is the acceleration from the noisy newest velocity to the next
+    frame's, so the network learns to pull a drifted state back."""
     torch = traj.torch
     idx = torch.cat([torch.arange(int(traj.off[k]), int(traj.off[k + 1]), device=traj.pos.device) for k in scenes])
     pos = traj.pos[frame, idx].contiguous()
-    vh = torch.stack([traj.vel[frame - t, idx] for t in range(HIST)]).contiguous()
+    vh = torch.stack([traj.vel[frame - t, idx] for t in range(HIST)])
     cls = traj.cls[idx].contiguous()
+    if noise > 0.0:
+        mv = (cls < 2).to(torch.float32)[None, :, None]
+        vh = vh + noise * mv * torch.randn(vh.shape, generator=gen, device=vh.device)
+    vh = vh.contiguous()
     sc = torch.cat([torch.full((int(traj.off[k + 1] - traj.off[k]),), q, dtype=torch.int32, device=pos.device)
                     for q, k in enumerate(scenes)])
-    tgt = ((traj.vel[frame + 1, idx] - traj.vel[frame, idx]) * (ascale / (traj.stride * traj.dt))).contiguous()
-    mask = (cls < 2).to(torch.int32)
-    return pos, vh, cls, sc, tgt, mask, idx
+    tgt = ((traj.vel[frame + 1, idx] - vh[0]) * (ascale / (traj.stride * traj.dt))).contiguous()
+    weight = torch.where(cls == 0, 1, torch.where(cls == 1, DEBRIS_WEIGHT, 0)).to(torch.int32)
+    return pos, vh, cls, sc, tgt, weight, idx
 
 
 def window_steps(start: int, gap: int, windows: int, stride: int = STRIDE) -> List[int]:
@@ -472,20 +488,22 @@ def window_steps(start: int, gap: int, windows: int, stride: int = STRIDE) -> Li
 
 
 def train(sur: Surrogate, traj: Trajectories, iters: int, batch: int = 4, lr: float = 1e-3, seed: int = 0,
-          log=None) -> List[float]:
+          log=None, noise: float = 0.0) -> List[float]:
     """Adam on random (scene batch, sample frame) pairs; cosine learning-rate decay."""
     rng = np.random.default_rng(seed)
+    gen = traj.torch.Generator(device=traj.pos.device)
+    gen.manual_seed(seed)
     losses = []
     for it in range(iters):
         sc = sorted(rng.choice(traj.nscene, size=min(batch, traj.nscene), replace=False).tolist())
         fr = int(traj.samples[int(rng.integers(len(traj.samples)))])
-        pos, vh, cls, scn, tgt, mask, _ = _batch(traj, sc, fr, sur.ascale)
+        pos, vh, cls, scn, tgt, wgt, _ = _batch(traj, sc, fr, sur.ascale, noise, gen)
         n = pos.shape[0]
         rp, ci, _ = sur.graph(pos, scn, len(sc))
         x0 = sur.features(vh, cls, pos)
         sur.forward(x0, pos, rp, ci, n)
         # targets in units of their standard deviation, clipped at 5 (DEM contact impulses are heavy-tailed)
-        losses.append(sur.loss_and_backward(x0, pos, rp, ci, n, tgt, mask, int(mask.sum().item()), clip=5.0))
+        losses.append(sur.loss_and_backward(x0, pos, rp, ci, n, tgt, wgt, int(wgt.sum().item()), clip=5.0))
         sur.adam(lr * 0.5 * (1.0 + np.cos(np.pi * it / max(iters, 1))))
         if log and (it % 50 == 0 or it == iters - 1):
             log("iter %d loss %.4f" % (it, losses[-1]))
@@ -551,6 +569,45 @@ def rollout(sur: Optional[Surrogate], traj: Trajectories, start: int, steps: int
 
 
 
+def advance_batch(sur: Surrogate, scenes, steps: int, dt: float) -> Dict[str, float]:
+    """Throughput: the surrogate alone advances a batch of scenes from their
+    initial state (velocity history = initial velocities) `steps` strides of
+    STRIDE * dt; returns the wall time per surrogate step and the node count."""
+    import torch
+    dev = sur.dev
+    pos = torch.tensor(np.concatenate([sc.pos for sc in scenes]).astype(np.float32), device=dev)
+    v0 = torch.tensor(np.concatenate([sc.vel for sc in scenes]).astype(np.float32), device=dev)
+    vh = v0.unsqueeze(0).repeat(HIST, 1, 1).contiguous()
+    typ = np.concatenate([sc.type for sc in scenes])
+    cls = torch.tensor(np.where(typ < 2, 0, np.where(typ < 4, 1, 2)).astype(np.int32), device=dev)
+    body, boff = [], 0
+    for sc in scenes:
+        b = sc.body if sc.body is not None else np.full(sc.n, -1, np.int32)
+        body.append(np.where(b >= 0, b + boff, -1).astype(np.int32))
+        boff += int(b.max()) + 1 if (b >= 0).any() else 0
+    body = torch.tensor(np.concatenate(body), device=dev)
+    scene = torch.tensor(np.repeat(np.arange(len(scenes), dtype=np.int32), [sc.n for sc in scenes]), device=dev)
+    body_sums = torch.zeros((max(boff, 1), 4), dtype=torch.float32, device=dev)
+    n = pos.shape[0]
+    h = STRIDE * dt
+
+    def one():
+        rp, ci, _ = sur.graph(pos, scene, len(scenes))
+        x0 = sur.features(vh, cls, pos, train=False)
+        y = sur.forward(x0, pos, rp, ci, n, train=False)
+        sur.rollout_step(pos, vh, y, cls, body, boff, h, body_sums)
+
+    one()   # warm-up (workspace, graph capacity)
+    torch.cuda.synchronize(dev)
+    t0 = _now()
+    for _ in range(steps):
+        one()
+    torch.cuda.synchronize(dev)
+    el = _now() - t0
+    ok = bool(torch.isfinite(pos).all().item())
+    return {"seconds_per_step": el / steps, "nodes": n, "finite": ok}
+
+
 def save(sur: Surrogate, path: str, **meta) -> None:
     sd = sur.state_dict()
     np.savez(path, vscale=sur.vscale, ascale=sur.ascale, d=sur.d, tank_lo=sur.tank_lo, tank_hi=sur.tank_hi,
@@ -594,6 +651,7 @@ def main(argv=None) -> int:
     ap.add_argument("--iters", type=int, default=1500)
     ap.add_argument("--batch", type=int, default=4)
     ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--noise", type=float, default=0.01, help="velocity-history noise in training (m/s)")
     ap.add_argument("--eval-start", type=int, default=2000, help="solver step where rollouts start")
     ap.add_argument("--eval-steps", type=int, default=100, help="surrogate steps (10 solver steps each)")
     ap.add_argument("--reanchor", type=int, default=200)
@@ -612,8 +670,8 @@ def main(argv=None) -> int:
         log("data %.1f s; vscale %.3g ascale %.3g" % (traj.sim_seconds, vs, asc))
         sur = Surrogate(d=0.01, tank_lo=[0, 0, 0], tank_hi=[0.8, 0.4, 0.6], vscale=vs, ascale=asc, seed=a.seed)
         t0 = _now()
-        losses = train(sur, traj, a.iters, a.batch, a.lr, a.seed, log)
-        rep["train"] = {"scenes": len(sc), "windows": a.windows, "iters": a.iters, "batch": a.batch,
+        losses = train(sur, traj, a.iters, a.batch, a.lr, a.seed, log, noise=a.noise)
+        rep["train"] = {"scenes": len(sc), "windows": a.windows, "iters": a.iters, "batch": a.batch, "noise": a.noise,
                         "seconds": _now() - t0, "loss_first": losses[0], "loss_last10": float(np.mean(losses[-10:]))}
         save(sur, a.out, **rep["train"])
         del traj

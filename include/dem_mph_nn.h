@@ -74,8 +74,9 @@ size_t dmph_nn_head_scratch(int N);
 int dmph_nn_head_bwd(const uint16_t *H, const float *Wd, const float *dY, uint16_t *dH, float *dWd, float *dbd, int N,
                      float *scratch, void *stream);
 
-/* mean squared error over masked nodes (count of them) against targets clipped to
- * [-clip, clip]: loss (device float), dY */
+/* weighted mean squared error: sum_i mask[i] |Y_i - clip(T_i)|^2 / count with
+ * integer weights mask[i] (0 = left out), targets clipped to [-clip, clip];
+ * loss (device float), dY */
 size_t dmph_nn_mse_scratch(int N);
 int dmph_nn_mse(const float *Y, const float *T, const int *mask, int N, int count, float clip, float *dY, float *loss,
                 float *scratch, void *stream);

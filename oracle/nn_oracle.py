@@ -67,8 +67,9 @@ def forward(P: Dict[str, np.ndarray], x0: np.ndarray, pos: np.ndarray, rp: np.nd
 
 
 def loss(y: np.ndarray, target: np.ndarray, mask: np.ndarray, count: int) -> float:
-    d = (y - target) * mask[:, None]
-    return float((d * d).sum() / count)
+    """sum_i mask[i] |y_i - target_i|^2 / count; mask holds per-node weights (0 = left out)."""
+    d = y - target
+    return float((mask[:, None] * d * d).sum() / count)
 
 
 def backward(P: Dict[str, np.ndarray], c, y: np.ndarray, target: np.ndarray, mask: np.ndarray,

@@ -112,7 +112,8 @@ def _small_problem(S, seed=0):
     vh = torch.tensor(rng.standard_normal((5, n, 3)).astype(np.float32) * 0.2, device="cuda:0")
     cls = torch.tensor(np.where(sc.type < 2, 0, np.where(sc.type < 4, 1, 2)).astype(np.int32), device="cuda:0")
     tgt = torch.tensor(rng.standard_normal((n, 3)).astype(np.float32), device="cuda:0")
-    mask = (cls < 2).to(torch.int32)
+    # the training weights: fluid 1, debris 5, walls 0
+    mask = torch.where(cls == 0, 1, torch.where(cls == 1, 5, 0)).to(torch.int32)
     return sur, sc, pos, vh, cls, tgt, mask
 
 
@@ -126,7 +127,7 @@ def test_network_gradients_match_fp64_oracle(S):
     x0 = sur.features(vh, cls, pos)
     y = sur.forward(x0, pos, rp, ci, n).clone()
     count = int(mask.sum().item())
-    sur.loss_and_backward(x0, pos, rp, ci, n, tgt, mask, count)
+    lg = sur.loss_and_backward(x0, pos, rp, ci, n, tgt, mask, count)
     P = sur.state_dict()
     X0 = _host_bf16(x0[:n])
     yr, cache = NO.forward(P, X0, pos.cpu().numpy().astype(np.float64), rp.cpu().numpy(), ci.cpu().numpy()[:nnz],
@@ -135,6 +136,8 @@ def test_network_gradients_match_fp64_oracle(S):
     Gr = NO.backward(P, cache, yr, tgt.cpu().numpy().astype(np.float64), m, count)
     ey = np.linalg.norm(y.cpu().numpy() - yr) / np.linalg.norm(yr)
     assert ey < 0.05, ey
+    lr_ = NO.loss(yr, tgt.cpu().numpy().astype(np.float64), m, count)
+    assert abs(lg - lr_) < 0.05 * lr_, (lg, lr_)
     G = sur.grads()
     worst = {}
     for k in Gr:

@@ -14,7 +14,8 @@ def _problem(seed=3, n=40):
     rp, ci = NO.radius_graph(pos, radius)
     x0 = rng.standard_normal((n, 32))
     target = rng.standard_normal((n, 3))
-    mask = (rng.uniform(size=n) < 0.7).astype(float)
+    # integer per-node loss weights as training uses them: 0 (left out), 1, 5
+    mask = rng.choice([0.0, 1.0, 5.0], size=n, p=[0.3, 0.5, 0.2])
     return pos, radius, rp, ci, x0, target, mask, int(mask.sum())
 
 
