@@ -133,7 +133,9 @@ def copy_probe(mib: int) -> None:
     kernel's FETCH_SIZE / WRITE_SIZE under rocprof calibrate the counters
     against a known byte count."""
     import ctypes as C
-    hip = C.CDLL("libamdhip64.so")
+    from dem_mph_amd import share_hip_runtime
+    share_hip_runtime()
+    hip = C.CDLL("libamdhip64.so.7")   # by soname: the runtime already loaded, if any
     n = mib << 20
     a, b = C.c_void_p(), C.c_void_p()
     assert hip.hipMalloc(C.byref(a), C.c_size_t(n)) == 0 and hip.hipMalloc(C.byref(b), C.c_size_t(n)) == 0

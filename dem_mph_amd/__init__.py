@@ -125,12 +125,27 @@ class Info(C.Structure):
 _lib = None
 
 
+def share_hip_runtime() -> None:
+    """Load PyTorch (when installed) before any library of this package.
+
+    PyTorch ships its own HIP runtime under the same soname as /opt/rocm's
+    (libamdhip64.so.7).  With torch loaded first, the libraries here bind to
+    that copy and the process has one runtime, so device pointers pass freely
+    between torch and the C ABI.  In the other order torch loads a second
+    runtime later, which then reports no GPU."""
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+
+
 def lib() -> C.CDLL:
     """Load libdemmph.so (fails loudly when it has not been built)."""
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise DmphError("libdemmph.so is not built; run `python -m dem_mph_amd.build`")
+        share_hip_runtime()
         L = C.CDLL(LIB_PATH)
         vp, ip, dp = C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_double)
         L.dmph_default_options.argtypes = [C.POINTER(Options)]

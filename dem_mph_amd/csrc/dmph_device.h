@@ -39,6 +39,7 @@ struct Geo {
     int p2, p2_log;     // bitonic size (power of two > n) and its log2
     int compensated;    // FP32 mode: FP64 accumulators for the pair sums
     int kcap;           // neighbour-list capacity per particle
+    int knear;          // FP32 rows: near segment capacity (far entries start at this position), a multiple of 16
     int lanes[2];       // FP32 sweeps 1 and 2: lanes per particle (2, 4, 8 or 16)
     int ccap;           // DEM contact-list capacity per particle
     double cw[3], dmin[3], dw[3];

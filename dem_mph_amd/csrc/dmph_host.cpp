@@ -300,21 +300,38 @@ class Engine final : public dmph_handle {
             err = "more than 67,108,863 particles";
             return DMPH_E_ARG;
         }
-        {   // neighbour-list capacity: 1.6 x the lattice count within MaxRadius + 0.1 d, rounded up to 16
-            const double rc = (g.rmax + 0.1 * g.d) / g.d;
-            const int rr = (int)rc + 1, zr = g.dim == 2 ? 0 : rr;
-            int cnt = 0;
-            for (int ix = -rr; ix <= rr; ix++)
-                for (int iy = -rr; iy <= rr; iy++)
-                    for (int iz = -zr; iz <= zr; iz++)
-                        if ((double)(ix * ix + iy * iy + iz * iz) <= rc * rc) cnt++;
-            g.kcap = ((int)ceil(1.6 * cnt) + 15) / 16 * 16;
-            if (getenv("DMPH_NEIGHBOR_CAPACITY")) g.kcap = (atoi(getenv("DMPH_NEIGHBOR_CAPACITY")) + 7) / 8 * 8;
-            g.ccap = 32;
+        {   // neighbour-list capacity: 1.6 x the lattice count within MaxRadius + 0.1 d, in
+            // multiples of 16; FP32 rows split it into a near and a far segment, each
+            // sized from its own lattice count
             // near segment (FP32): every pass but the viscosity and surface-tension
             // ones stays within RadiusP; the 0.1 d skin keeps it conservative
             const double rn = (g.st_active ? std::max(g.rp, g.ra) : g.rp) + 0.1 * g.d;
             g.near2 = rn * rn;
+            const double rc = (g.rmax + 0.1 * g.d) / g.d, rnd = rn / g.d;
+            const int rr = (int)std::max(rc, rnd) + 1, zr = g.dim == 2 ? 0 : rr;
+            int cnt = 0, cnn = 0;
+            for (int ix = -rr; ix <= rr; ix++)
+                for (int iy = -rr; iy <= rr; iy++)
+                    for (int iz = -zr; iz <= zr; iz++) {
+                        const double q = (double)(ix * ix + iy * iy + iz * iz);
+                        cnt += q <= rc * rc;
+                        cnn += q <= std::min(rc, rnd) * std::min(rc, rnd);
+                    }
+            auto up16 = [](double x) { return ((int)ceil(x) + 15) / 16 * 16; };
+            if (std::is_same<R, double>::value) {
+                g.kcap = g.knear = up16(1.6 * cnt);
+            } else {
+                g.knear = up16(1.6 * cnn);
+                g.kcap = g.knear + up16(1.6 * (cnt - cnn));
+            }
+            if (getenv("DMPH_NEIGHBOR_CAPACITY")) {
+                const int total = std::max(32, up16(atoi(getenv("DMPH_NEIGHBOR_CAPACITY"))));
+                g.kcap = total;
+                g.knear = std::is_same<R, double>::value
+                              ? total
+                              : std::min(total - 16, std::max(16, up16((double)total * cnn / std::max(cnt, 1))));
+            }
+            g.ccap = 32;
             // contact list: within d now means within d + 0.1 d at rebinning; a further
             // 0.15 d covers the solids' motion over the DEM sub-steps of one step
             g.con2 = (1.25 * g.d) * (1.25 * g.d);
@@ -670,8 +687,8 @@ class Engine final : public dmph_handle {
         HCHK(hipMemcpy(&ov, &a.sc->list_overflow, sizeof ov, hipMemcpyDeviceToHost));
         if (ov > 0) {
             char b[200];
-            snprintf(b, sizeof b, "neighbour list capacity %d exceeded by a row of %d entries "
-                     "(set DMPH_NEIGHBOR_CAPACITY); results after that rebinning are invalid", g.kcap, ov);
+            snprintf(b, sizeof b, "neighbour list capacity %d (near segment %d) exceeded by a row of %d entries "
+                     "(set DMPH_NEIGHBOR_CAPACITY); results after that rebinning are invalid", g.kcap, g.knear, ov);
             err = b;
             return DMPH_E_STATE;
         }
@@ -901,10 +918,10 @@ class Engine final : public dmph_handle {
         } else {
             TRY(download(nn, a.nnear, n));
         }
-        // FP32 rows: near entries at the front, far entries at the back of the capacity
+        // FP32 rows: near segment from position 0, far segment from knear
         for (int i = 0; i < n; i++)
             for (int k = 0; k < cnt[i]; k++) {
-                const int q = k < nn[i] ? k : k + (g.kcap - cnt[i]);
+                const int q = k < nn[i] ? k : g.knear + (k - nn[i]);
                 idx[pt[i] + k] = (int)(list[((size_t)(q >> 3) * n + i) * 8 + (q & 7)] >> 6);
             }
         return DMPH_OK;

@@ -60,6 +60,8 @@ def nnlib():
     if _nn is None:
         if not os.path.exists(NN_LIB):
             raise RuntimeError("%s is not built (python -m dem_mph_amd.build)" % NN_LIB)
+        from . import share_hip_runtime
+        share_hip_runtime()
         _nn = C.CDLL(NN_LIB)
         for f in ("dmph_nn_dw_scratch", "dmph_nn_edge_scratch", "dmph_nn_head_scratch", "dmph_nn_mse_scratch"):
             getattr(_nn, f).restype = C.c_size_t
