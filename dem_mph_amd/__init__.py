@@ -19,7 +19,7 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "libdemmph.so")
 
 NTYPE = 6
-OK, E_ARG, E_IO, E_FORMAT, E_DOMAIN, E_DEVICE, E_STATE = 0, -1, -2, -3, -4, -5, -6
+OK, E_ARG, E_IO, E_FORMAT, E_DOMAIN, E_DEVICE, E_STATE, E_ALLOC = 0, -1, -2, -3, -4, -5, -6, -7
 PREC_FP64, PREC_FP32 = 0, 1
 SORT_CANONICAL, SORT_REFERENCE = 0, 1
 REBUILD_SKIN, REBUILD_EVERY_STEP = 0, 1
@@ -39,6 +39,7 @@ EXPORTED = ("dmph_default_options", "dmph_params_read", "dmph_grid_read", "dmph_
             "dmph_get_bodies", "dmph_get_info",
             "dmph_virial", "dmph_pairs", "dmph_write_prof", "dmph_write_vtk", "dmph_set_timing", "dmph_get_timing",
             "dmph_autotune", "dmph_export_state",
+            "dmph_boid_generate", "dmph_boid_error", "dmph_grid_write",
             "dmph_last_error", "dmph_destroy")
 
 TIMING_GROUPS = ("predict", "rebin", "sweep1", "sweep2", "accel", "dem", "move", "end")
@@ -151,6 +152,9 @@ def lib() -> C.CDLL:
         L.dmph_default_options.argtypes = [C.POINTER(Options)]
         L.dmph_params_read.argtypes = [C.c_char_p, C.POINTER(Params)]
         L.dmph_grid_read.argtypes = [C.c_char_p, C.POINTER(Grid)]
+        L.dmph_boid_generate.argtypes = [C.c_char_p, C.POINTER(Grid)]
+        L.dmph_boid_error.restype = C.c_char_p
+        L.dmph_grid_write.argtypes = [C.c_char_p, C.POINTER(Grid)]
         L.dmph_grid_free.argtypes = [C.POINTER(Grid)]
         L.dmph_create.argtypes = [C.POINTER(Params), C.POINTER(Options), C.POINTER(Grid), C.POINTER(vp)]
         L.dmph_step.argtypes = [vp, C.c_int]
@@ -169,7 +173,7 @@ def lib() -> C.CDLL:
         L.dmph_last_error.restype = C.c_char_p
         L.dmph_destroy.argtypes = [vp]
         for f in EXPORTED:
-            if f not in ("dmph_default_options", "dmph_grid_free", "dmph_destroy", "dmph_last_error"):
+            if f not in ("dmph_default_options", "dmph_grid_free", "dmph_destroy", "dmph_last_error", "dmph_boid_error"):
                 getattr(L, f).restype = C.c_int
         _lib = L
     return _lib
@@ -183,23 +187,46 @@ def default_options(**kw) -> Options:
     return o
 
 
+def _scene_of(g: "Grid"):
+    from . import scenes
+    n = g.n
+    t = np.ctypeslib.as_array(g.type, (n,)).astype(np.int32).copy() if n else np.zeros(0, np.int32)
+    pos = np.ctypeslib.as_array(g.pos, (n, 3)).copy() if n else np.zeros((0, 3))
+    vel = np.ctypeslib.as_array(g.vel, (n, 3)).copy() if n else np.zeros((0, 3))
+    body = np.ctypeslib.as_array(g.body, (n,)).astype(np.int32).copy() if n and bool(g.body) else None
+    dens = np.ctypeslib.as_array(g.density, (n,)).copy() if n and bool(g.density) else None
+    dmin, dmax = np.array(g.domain_min[:]), np.array(g.domain_max[:])
+    return scenes.Scene(d=g.spacing, dim=3, domain_min=dmin, domain_max=dmax, type=t, pos=pos, vel=vel,
+                        body=body, density=dens, time=g.time)
+
+
+def generate_boid(path: str, grid_path: Optional[str] = None):
+    """The reference's scene generator (generator/generator.cpp): a .boid file
+    into a scenes.Scene, and into a .grid file in the generator's layout when
+    `grid_path` is given.  Raises on the generator's parse errors (the shapes
+    read before the error are what the generator would still write)."""
+    g = Grid()
+    rc = lib().dmph_boid_generate(path.encode(), C.byref(g))
+    try:
+        if rc != OK:
+            raise DmphError("%s: %s (code %d)" % (path, lib().dmph_boid_error().decode(), rc))
+        if grid_path is not None:
+            w = lib().dmph_grid_write(grid_path.encode(), C.byref(g))
+            if w != OK:
+                raise DmphError("cannot write %s (code %d)" % (grid_path, w))
+        return _scene_of(g)
+    finally:
+        lib().dmph_grid_free(C.byref(g))
+
+
 def read_grid(path: str):
     """The reference's readGridFile (source/DEM-MPH.cpp:681-851) into a scenes.Scene."""
-    from . import scenes
     g = Grid()
     rc = lib().dmph_grid_read(path.encode(), C.byref(g))
     if rc != OK:
         raise DmphError("cannot read %s (code %d)" % (path, rc))
     try:
-        n = g.n
-        t = np.ctypeslib.as_array(g.type, (n,)).astype(np.int32).copy()
-        pos = np.ctypeslib.as_array(g.pos, (n, 3)).copy()
-        vel = np.ctypeslib.as_array(g.vel, (n, 3)).copy()
-        body = np.ctypeslib.as_array(g.body, (n,)).astype(np.int32).copy() if bool(g.body) else None
-        dens = np.ctypeslib.as_array(g.density, (n,)).copy() if bool(g.density) else None
-        dmin, dmax = np.array(g.domain_min[:]), np.array(g.domain_max[:])
-        return scenes.Scene(d=g.spacing, dim=3, domain_min=dmin, domain_max=dmax, type=t, pos=pos, vel=vel,
-                            body=body, density=dens, time=g.time)
+        return _scene_of(g)
     finally:
         lib().dmph_grid_free(C.byref(g))
 

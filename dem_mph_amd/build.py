@@ -20,6 +20,7 @@ BUILD = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "libdemmph.so")
 NN_LIB = os.path.join(PKG, "libdemmph_nn.so")
 CLI = os.path.join(PKG, "DEM-MPH")
+GEN_CLI = os.path.join(PKG, "GeneratorForMph")
 ARCH = os.environ.get("DMPH_OFFLOAD_ARCH", "gfx950")
 
 
@@ -57,6 +58,9 @@ def build(verbose: bool = False) -> str:
          os.path.join(CSRC, "dmph_host.cpp"), "-o", os.path.join(BUILD, "dmph_host.o")],
         [hipcc, "-x", "c++", *COMMON, "-ffp-contract=off", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include", "-c",
          os.path.join(CSRC, "dmph_io.cpp"), "-o", os.path.join(BUILD, "dmph_io.o")],
+        # scene generator (.boid -> .grid): host code, the generator's arithmetic unfused
+        [hipcc, "-x", "c++", *COMMON, "-ffp-contract=off", "-c", os.path.join(CSRC, "dmph_boid.cpp"),
+         "-o", os.path.join(BUILD, "dmph_boid.o")],
         # surrogate network kernels (bf16 MFMA), a library of their own
         [hipcc, "-x", "hip", *COMMON, *DEVICE, "-ffp-contract=fast", "-c", os.path.join(CSRC, "nn", "dmph_nn.hip"),
          "-o", os.path.join(BUILD, "dmph_nn.o")],
@@ -65,7 +69,7 @@ def build(verbose: bool = False) -> str:
         for r in ex.map(_run, jobs):
             if verbose and (r.stdout or r.stderr):
                 print(r.stdout, r.stderr, file=sys.stderr)
-    objs = [os.path.join(BUILD, o) for o in ("dmph_k64.o", "dmph_k32.o", "dmph_host.o", "dmph_io.o")]
+    objs = [os.path.join(BUILD, o) for o in ("dmph_k64.o", "dmph_k32.o", "dmph_host.o", "dmph_io.o", "dmph_boid.o")]
     tmp = LIB + ".tmp"
     _run([hipcc, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp, *objs])
     os.replace(tmp, LIB)
@@ -74,6 +78,9 @@ def build(verbose: bool = False) -> str:
     _run([hipcc, "-x", "c++", *COMMON, "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include", os.path.join(CSRC, "dmph_cli.cpp"), "-o", CLI + ".tmp",
           "-L" + PKG, "-ldemmph", "-Wl,-rpath,$ORIGIN"])
     os.replace(CLI + ".tmp", CLI)
+    _run([hipcc, "-x", "c++", *COMMON, os.path.join(CSRC, "dmph_gen_cli.cpp"), "-o", GEN_CLI + ".tmp",
+          "-L" + PKG, "-ldemmph", "-Wl,-rpath,$ORIGIN"])
+    os.replace(GEN_CLI + ".tmp", GEN_CLI)
     return LIB
 
 

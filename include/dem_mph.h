@@ -27,7 +27,8 @@ enum {
     DMPH_E_FORMAT = -3,    /* malformed .grid/.data content */
     DMPH_E_DOMAIN = -4,    /* cell grid unusable (fewer than 3 cells on an axis) */
     DMPH_E_DEVICE = -5,    /* HIP runtime error or no device */
-    DMPH_E_STATE = -6      /* call not valid in the handle's current state */
+    DMPH_E_STATE = -6,     /* call not valid in the handle's current state */
+    DMPH_E_ALLOC = -7      /* host memory exhausted */
 };
 
 /* Parameters: the keys of the reference's .data file (source/DEM-MPH.cpp:634-660). */
@@ -86,6 +87,25 @@ int dmph_params_read(const char *path, dmph_params *out);
  * release them with dmph_grid_free. */
 int dmph_grid_read(const char *path, dmph_grid *out);
 void dmph_grid_free(dmph_grid *g);
+
+/* The scene generator (generator/generator.cpp): readfile (:129-183) and the
+ * block readers (:187-658) parse a .boid file, genparticle (:660-944) fills
+ * *out (time 0, spacing = ParticleDistance or -1, domain = Lower/UpperDomain,
+ * body and density NULL) kind by kind -- Cuboid, Cuboid2, Cyboid, Cyboid2,
+ * Recboid, Recboid2 -- with the generator's arithmetic.  Release with
+ * dmph_grid_free.  DMPH_E_FORMAT on the generator's parse errors and on end of
+ * file inside a block (where the generator hangs), DMPH_E_IO when the file
+ * cannot be opened; either way *out holds the shapes read before the error,
+ * as the generator would write them.  dmph_boid_error() describes the last
+ * failure of the calling thread. */
+int dmph_boid_generate(const char *path, dmph_grid *out);
+const char *dmph_boid_error(void);
+
+/* writefile (generator/generator.cpp:948-974): the generator's .grid layout
+ * (Time "%lf", header "%d %e  %e %e %e  %e %e %e", then
+ * "%d    %e %e %e  %e %e %e " per particle), which dmph_grid_read and the
+ * reference's readGridFile read back. */
+int dmph_grid_write(const char *path, const dmph_grid *g);
 
 typedef struct dmph_handle dmph_handle;
 
