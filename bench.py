@@ -335,7 +335,9 @@ def main(argv=None) -> int:
     units = {"sweep1": n, "sweep2": info.solid_end - info.fluid_begin, "dem": info.wall_end - info.solid_begin}[dom]
     avg_s = ms / 1e3 / launches
     achieved = ALGO_BYTES[dom] * units / avg_s / 1e9
-    kname = {"sweep1": "k_sweep1f<", "sweep2": "k_sweep2f<", "dem": "k_dem<float"}[dom]
+    # the kernels the autotuner picked (the profile holds every candidate's launches)
+    lanes = mixed["lanes"]
+    kname = {"sweep1": "k_sweep1f<%d, " % lanes[0], "sweep2": "k_sweep2f<%d, " % lanes[1], "dem": "k_dem<float"}[dom]
     out = {
         "metric": METRIC,
         "value": pss,

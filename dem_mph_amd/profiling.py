@@ -90,6 +90,8 @@ def main(argv=None) -> int:
     ap.add_argument("--json")
     ap.add_argument("--top", type=int, default=14)
     ap.add_argument("--counters", help="a --pmc database with SQ_ counters (VALU utilisation)")
+    ap.add_argument("--l1", help="a --pmc database with TCP_TOTAL_CACHE_ACCESSES_sum and TCP_TCC_READ_REQ_sum")
+    ap.add_argument("--cus", type=int, default=256, help="CUs on the device (one L1 each)")
     ap.add_argument("--sclk-mhz", type=float, default=2400.0, help="shader clock for the utilisation estimate")
     ap.add_argument("--simds", type=int, default=1024, help="SIMDs on the device (256 CUs x 4)")
     a = ap.parse_args(argv)
@@ -132,6 +134,29 @@ def main(argv=None) -> int:
                 name, w, c.get("SQ_INSTS_VALU", 0) / w, c.get("SQ_INSTS_VMEM_RD", 0) / w, c.get("SQ_INSTS_LDS", 0) / w,
                 util))
             js["counters"][name] = dict(c, valu_issue_util=util)
+    if a.l1:
+        pl = pmc_means(a.l1)
+        pq = pmc_means(a.counters) if a.counters else {}
+        lines += ["", "Vector L1 (TCP) per dispatch (one `--pmc` pass): tag lookups, lookups per vector-memory "
+                      "instruction (SQ_INSTS_VMEM_RD from the counter pass above), misses sent to L2, and the lookup "
+                      "rate against one lookup per clock per CU (%d CUs at %.0f MHz):" % (a.cus, a.sclk_mhz), "",
+                  "| kernel | L1 lookups (M) | lookups / VMEM instr | to L2 (M) | L1 lookup util |", "|---|---|---|---|---|"]
+        js["l1"] = {}
+        for name, n, avg, tot in st[: a.top]:
+            c = pl.get(name)
+            if not c or "TCP_TOTAL_CACHE_ACCESSES_sum" not in c:
+                continue
+            acc = c["TCP_TOTAL_CACHE_ACCESSES_sum"]
+            cyc = c.get("duration_ns", 0.0) * a.sclk_mhz / 1e3
+            util = acc / (a.cus * cyc) if cyc > 0 else 0.0
+            q = pq.get(name, {})
+            vm = q.get("SQ_INSTS_VMEM_RD", 0.0)
+            per = acc / vm if vm > 0 else float("nan")
+            if acc < 1e5:
+                continue
+            lines.append("| `%s` | %.1f | %.1f | %.2f | %.2f |" % (name, acc / 1e6, per, c.get("TCP_TCC_READ_REQ_sum", 0.0) / 1e6,
+                                                                 util))
+            js["l1"][name] = dict(c, lookups_per_vmem=per, l1_lookup_util=util)
     open(a.out, "w").write("\n".join(lines) + "\n")
     if a.json:
         json.dump(js, open(a.json, "w"), indent=1, sort_keys=True)
