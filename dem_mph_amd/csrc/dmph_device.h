@@ -41,6 +41,7 @@ struct Geo {
     int kcap;           // neighbour-list capacity per particle
     int knear;          // FP32 rows: near segment capacity (far entries start at this position), a multiple of 16
     int lanes[2];       // FP32 sweeps 1 and 2: lanes per particle (2, 4, 8 or 16)
+    int fuse_accel;     // FP32: the fluid sweep 2 also applies calculateAcceleration (0 while re-running sweeps)
     int ccap;           // DEM contact-list capacity per particle
     double cw[3], dmin[3], dw[3];
     double d, vol, dt, dem_dt;
@@ -88,11 +89,13 @@ struct Dev {
     int *key_new, *order, *perm;   // [n]
     int *bkey, *bval;              // [p2] bitonic
     int *block_sums;               // scan carries
-    int *t_int;                    // [4n] gather scratch
+    int *t_int;                    // [5n] gather scratch
     R *t_real;                     // [8n]
     double *t_dbl;                 // [3n]
     // rigid bodies
-    double *b_mass, *b_x, *b_v, *b_l, *b_q, *b_iinv, *b_f, *b_t, *b_w, *b_r;
+    double *b_mass, *b_x, *b_v, *b_l, *b_q, *b_iinv, *b_w, *b_r;
+    int *mrank;                    // [n] rank of a member within its body (fixed at set-up, moves with the particle)
+    int *b_moff, *b_mslot;         // per body: offsets into b_mslot [nb + 1]; member slots [members], by rank
     double *blk_max;               // per-block maxima of the squared displacement
     float4 *p4, *v4;               // FP32 sweeps: packed (position, type) and (velocity, PressureP)
     Scalars *sc;

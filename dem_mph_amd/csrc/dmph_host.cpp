@@ -276,6 +276,7 @@ class Engine final : public dmph_handle {
         g.substeps = (int)(p.dt / p.dem_dt);   // source/DEM-MPH.cpp:434
         g.compensated = o.compensated;
         g.lanes[0] = g.lanes[1] = (o.lanes == 2 || o.lanes == 4 || o.lanes == 16) ? o.lanes : 8;
+        g.fuse_accel = 1;
         g.sort_reference = o.sort_mode == DMPH_SORT_REFERENCE;
         g.rebuild_every = o.rebuild == DMPH_REBUILD_EVERY_STEP;
         weights(g, p);
@@ -370,7 +371,7 @@ class Engine final : public dmph_handle {
         TRY(alloc(a.key_new, n)); TRY(alloc(a.order, n)); TRY(alloc(a.perm, n));
         if (g.sort_reference) { TRY(alloc(a.bkey, (size_t)g.p2)); TRY(alloc(a.bval, (size_t)g.p2)); }
         TRY(alloc(a.block_sums, nk / 1024 + 2));
-        TRY(alloc(a.t_int, 4 * n)); TRY(alloc(a.t_real, 8 * n));
+        TRY(alloc(a.t_int, 5 * n)); TRY(alloc(a.t_real, 8 * n));
         TRY(alloc(a.t_dbl, g.rigid ? 3 * n : 1)); TRY(alloc(a.r0, g.rigid ? 3 * n : 1));
         TRY(alloc(a.sc, 1));
         TRY(alloc(a.blk_max, (size_t)(std::max(g.n, g.nb) + 255) / 256 + 1));
@@ -494,8 +495,18 @@ class Engine final : public dmph_handle {
                 W[3 * b + k] = Iinv[9 * b + 3 * k] * L[3 * b] + Iinv[9 * b + 3 * k + 1] * L[3 * b + 1] + Iinv[9 * b + 3 * k + 2] * L[3 * b + 2];
         }
         TRY(alloc(a.b_mass, nb)); TRY(alloc(a.b_x, 3 * nb)); TRY(alloc(a.b_v, 3 * nb)); TRY(alloc(a.b_l, 3 * nb));
-        TRY(alloc(a.b_q, 4 * nb)); TRY(alloc(a.b_iinv, 9 * nb)); TRY(alloc(a.b_f, 3 * nb)); TRY(alloc(a.b_t, 3 * nb));
+        TRY(alloc(a.b_q, 4 * nb)); TRY(alloc(a.b_iinv, 9 * nb));
         TRY(alloc(a.b_w, 3 * nb)); TRY(alloc(a.b_r, 9 * nb));
+        // member map: each member's rank in its body (input order), the body's
+        // offsets; k_copyback fills the slots of the ranks after every rebinning
+        std::vector<int> mrank(n, -1), moff(nb + 1, 0);
+        for (int i = 0; i < n; i++)
+            if (body[i] >= 0) mrank[i] = moff[body[i] + 1]++;
+        for (int b = 0; b < nb; b++) moff[b + 1] += moff[b];
+        TRY(alloc(a.mrank, n)); TRY(alloc(a.b_moff, nb + 1)); TRY(alloc(a.b_mslot, std::max(moff[nb], 1)));
+        HCHK(hipMemcpyAsync(a.mrank, mrank.data(), n * sizeof(int), hipMemcpyHostToDevice, stream));
+        HCHK(hipMemcpyAsync(a.b_moff, moff.data(), (nb + 1) * sizeof(int), hipMemcpyHostToDevice, stream));
+        HCHK(hipStreamSynchronize(stream));   // the host vectors go out of scope
         body_mass_host = M;
         auto up = [&](double *dst, const std::vector<double> &src) -> int {
             HCHK(hipMemcpyAsync(dst, src.data(), src.size() * sizeof(double), hipMemcpyHostToDevice, stream));
@@ -571,7 +582,9 @@ class Engine final : public dmph_handle {
         timed(DMPH_TIMING_REBIN, [&] { launch_rebin<R>(g, a, stream); });
         timed(DMPH_TIMING_SWEEP1, [&] { launch_sweep1<R>(g, a, stream); });
         timed(DMPH_TIMING_SWEEP2, [&] { launch_sweep2<R>(g, a, stream); });
-        timed(DMPH_TIMING_ACCEL, [&] { launch_accel<R>(g, a, stream); });
+        // FP32 applies the fluid acceleration inside sweep 2: no group of its own
+        if (std::is_same<R, double>::value || !g.fuse_accel)
+            timed(DMPH_TIMING_ACCEL, [&] { launch_accel<R>(g, a, stream); });
         for (int s = 0; s < g.substeps; s++) {
             timed(DMPH_TIMING_DEM, [&] { launch_dem<R>(g, a, stream); });
             timed(DMPH_TIMING_MOVE, [&] { launch_move<R>(g, a, stream); });
@@ -607,6 +620,7 @@ class Engine final : public dmph_handle {
         }
         reps = reps < 1 ? 1 : reps;
         const size_t n = g.n;
+        g.fuse_accel = 0;     // the re-runs must not advance the fluid velocities
         R *save = a.t_real;   // rebin scratch, free between steps
         HCHK(hipMemcpyAsync(save, a.fx, n * sizeof(R), hipMemcpyDeviceToDevice, stream));
         HCHK(hipMemcpyAsync(save + n, a.fy, n * sizeof(R), hipMemcpyDeviceToDevice, stream));
@@ -649,6 +663,7 @@ class Engine final : public dmph_handle {
         HCHK(hipMemcpyAsync(a.fx, save, n * sizeof(R), hipMemcpyDeviceToDevice, stream));
         HCHK(hipMemcpyAsync(a.fy, save + n, n * sizeof(R), hipMemcpyDeviceToDevice, stream));
         HCHK(hipMemcpyAsync(a.fz, save + 2 * n, n * sizeof(R), hipMemcpyDeviceToDevice, stream));
+        g.fuse_accel = 1;
         HCHK(hipStreamSynchronize(stream));
         HCHK(hipGetLastError());
         // launch arguments changed: capture the step again at the next call
