@@ -42,6 +42,7 @@ struct Geo {
     int knear;          // FP32 rows: near segment capacity (far entries start at this position), a multiple of 16
     int lanes[2];       // FP32 sweeps 1 and 2: lanes per particle (2, 4, 8 or 16)
     int fuse_accel;     // FP32: the fluid sweep 2 also applies calculateAcceleration (0 while re-running sweeps)
+    int nloose;         // solid particles outside rigid bodies (moved by k_move_solids)
     int ccap;           // DEM contact-list capacity per particle
     double cw[3], dmin[3], dw[3];
     double d, vol, dt, dem_dt;

@@ -500,8 +500,11 @@ class Engine final : public dmph_handle {
         // member map: each member's rank in its body (input order), the body's
         // offsets; k_copyback fills the slots of the ranks after every rebinning
         std::vector<int> mrank(n, -1), moff(nb + 1, 0);
-        for (int i = 0; i < n; i++)
+        g.nloose = 0;
+        for (int i = 0; i < n; i++) {
             if (body[i] >= 0) mrank[i] = moff[body[i] + 1]++;
+            else g.nloose += is_solid(gr.type[i]);
+        }
         for (int b = 0; b < nb; b++) moff[b + 1] += moff[b];
         TRY(alloc(a.mrank, n)); TRY(alloc(a.b_moff, nb + 1)); TRY(alloc(a.b_mslot, std::max(moff[nb], 1)));
         HCHK(hipMemcpyAsync(a.mrank, mrank.data(), n * sizeof(int), hipMemcpyHostToDevice, stream));
