@@ -18,6 +18,7 @@
 //
 // All entry points are extern "C" with plain device pointers and a stream.
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include <stdint.h>
 
 #include "dem_mph_nn.h"
@@ -248,28 +249,58 @@ __device__ __forceinline__ void edge_eps(const float *pos, int i, int j, float i
     e[3] = sqrtf(dx * dx + dy * dy + dz * dz) * inv_r;
 }
 
+// Edges are taken 16 at a time: lanes 0..15 load the batch's neighbour indices
+// and edge features, which are broadcast (readlane: uniform, in scalar
+// registers), and all 16 B-row loads are in flight before the arithmetic -- one
+// memory round trip per 16 edges instead of a dependent chain per edge.  The
+// sum still runs over the row in order.
+constexpr int kEdgeBatch = 16;
+
+__device__ __forceinline__ int rl(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
+__device__ __forceinline__ float rlf(float v, int l) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l)); }
+
 __global__ void __launch_bounds__(256) k_edge_fwd(const bf16_t *__restrict__ AB, const float *__restrict__ pos,
                                                   const int *__restrict__ rp, const int *__restrict__ ci,
                                                   const float *__restrict__ Wc, const float *__restrict__ bc,
-                                                  bf16_t *__restrict__ S, int N, float inv_r) {
+                                                  bf16_t *__restrict__ S, int lds, int N, float inv_r) {
     const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (i >= N) return;
-    const int f = 2 * (threadIdx.x & 63);
+    const int lane = threadIdx.x & 63, f = 2 * lane;
     const float a0 = bf2f(AB[(size_t)i * 256 + f]), a1 = bf2f(AB[(size_t)i * 256 + f + 1]);
     const float4 w0 = *reinterpret_cast<const float4 *>(Wc + 4 * f), w1 = *reinterpret_cast<const float4 *>(Wc + 4 * f + 4);
     const float b0 = bc[f], b1 = bc[f + 1];
+    const float xi = pos[3 * i], yi = pos[3 * i + 1], zi = pos[3 * i + 2];
     float s0 = 0.f, s1 = 0.f;
-    for (int q = rp[i]; q < rp[i + 1]; q++) {
-        const int j = ci[q];
-        float e[4];
-        edge_eps(pos, i, j, inv_r, e);
-        const uint32_t bj = *reinterpret_cast<const uint32_t *>(AB + (size_t)j * 256 + 128 + f);
-        const float c0 = a0 + bf2f((bf16_t)(bj & 0xffff)) + b0 + w0.x * e[0] + w0.y * e[1] + w0.z * e[2] + w0.w * e[3];
-        const float c1 = a1 + bf2f((bf16_t)(bj >> 16)) + b1 + w1.x * e[0] + w1.y * e[1] + w1.z * e[2] + w1.w * e[3];
-        s0 += fmaxf(c0, 0.f);
-        s1 += fmaxf(c1, 0.f);
+    const int q0 = rp[i], q1 = rp[i + 1];
+    for (int q = q0; q < q1; q += kEdgeBatch) {
+        const int m = min(kEdgeBatch, q1 - q);
+        // lane l < m: edge q + l
+        int jl = i;
+        float e0 = 0.f, e1 = 0.f, e2 = 0.f, e3 = 0.f;
+        if (lane < m) {
+            jl = ci[q + lane];
+            const float dx = pos[3 * jl] - xi, dy = pos[3 * jl + 1] - yi, dz = pos[3 * jl + 2] - zi;
+            e0 = dx * inv_r; e1 = dy * inv_r; e2 = dz * inv_r;
+            e3 = sqrtf(dx * dx + dy * dy + dz * dz) * inv_r;
+        }
+        uint32_t bj[kEdgeBatch];
+#pragma unroll
+        for (int t = 0; t < kEdgeBatch; t++) {
+            const int j = rl(jl, t);   // i itself past the row's end: a valid row, never summed
+            bj[t] = *reinterpret_cast<const uint32_t *>(AB + (size_t)j * 256 + 128 + f);
+        }
+#pragma unroll
+        for (int t = 0; t < kEdgeBatch; t++) {
+            if (t < m) {
+                const float x0 = rlf(e0, t), x1 = rlf(e1, t), x2 = rlf(e2, t), x3 = rlf(e3, t);
+                const float c0 = a0 + bf2f((bf16_t)(bj[t] & 0xffff)) + b0 + w0.x * x0 + w0.y * x1 + w0.z * x2 + w0.w * x3;
+                const float c1 = a1 + bf2f((bf16_t)(bj[t] >> 16)) + b1 + w1.x * x0 + w1.y * x1 + w1.z * x2 + w1.w * x3;
+                s0 += fmaxf(c0, 0.f);
+                s1 += fmaxf(c1, 0.f);
+            }
+        }
     }
-    *reinterpret_cast<uint32_t *>(S + (size_t)i * 128 + f) = (uint32_t)f2bf(s0) | ((uint32_t)f2bf(s1) << 16);
+    *reinterpret_cast<uint32_t *>(S + (size_t)i * lds + f) = (uint32_t)f2bf(s0) | ((uint32_t)f2bf(s1) << 16);
 }
 
 // dS: [N][128] FP32 (gradient of S).  Outputs dAB [N][256] bf16 and per-block
@@ -345,6 +376,79 @@ __global__ void k_g_keys(const float *pos, const int *scene, int N, NnGrid gd, i
     const int k = (scene ? scene[i] : 0) * gd.ncell + (c[0] * gd.cc[1] + c[1]) * gd.cc[2] + c[2];
     key[i] = k;
     atomicAdd(&counts[k], 1);
+}
+
+// Exclusive scan of n ints into out[0..n] (out[n] = total) in three passes over
+// tiles of 1024: tile sums, one block scanning the sums, then each tile's scan
+// from its carry.  The order of every addition is fixed (deterministic).
+constexpr int kTile = 1024;
+
+__global__ void __launch_bounds__(256) k_scan_tile_sums(const int *__restrict__ in, int n, int *__restrict__ sums) {
+    __shared__ int ws[4];
+    const int base = blockIdx.x * kTile + threadIdx.x * 4;
+    int v = 0;
+    for (int k = 0; k < 4; k++) v += base + k < n ? in[base + k] : 0;
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) sums[blockIdx.x] = ws[0] + ws[1] + ws[2] + ws[3];
+}
+
+__global__ void __launch_bounds__(256) k_scan_tiles(const int *__restrict__ in, int *__restrict__ out, int n,
+                                                    const int *__restrict__ carry) {
+    __shared__ int ws[4];
+    const int base = blockIdx.x * kTile + threadIdx.x * 4;
+    int v[4];
+    for (int k = 0; k < 4; k++) v[k] = base + k < n ? in[base + k] : 0;
+    const int loc = v[0] + v[1] + v[2] + v[3];
+    int inc = loc;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(inc, o);
+        if (lane >= o) inc += y;
+    }
+    if (lane == 63) ws[wid] = inc;
+    __syncthreads();
+    int run = carry[blockIdx.x] + inc - loc;
+    for (int w = 0; w < wid; w++) run += ws[w];
+    for (int k = 0; k < 4; k++) {
+        if (base + k < n) out[base + k] = run;
+        run += v[k];
+    }
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == blockDim.x - 1) out[n] = run;   // the total
+}
+
+// bounding box of pos [N][3]: per-block min / max, then one block over the blocks
+__global__ void __launch_bounds__(256) k_bbox_part(const float *__restrict__ pos, int N, float *__restrict__ part) {
+    __shared__ float ws[4][6];
+    float m[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x)
+        for (int k = 0; k < 3; k++) {
+            const float x = pos[3 * (size_t)i + k];
+            m[k] = fminf(m[k], x);
+            m[3 + k] = fmaxf(m[3 + k], x);
+        }
+    for (int o = 32; o > 0; o >>= 1)
+        for (int k = 0; k < 3; k++) {
+            m[k] = fminf(m[k], __shfl_xor(m[k], o));
+            m[3 + k] = fmaxf(m[3 + k], __shfl_xor(m[3 + k], o));
+        }
+    if ((threadIdx.x & 63) == 0)
+        for (int k = 0; k < 6; k++) ws[threadIdx.x >> 6][k] = m[k];
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        float r = ws[0][threadIdx.x];
+        for (int w = 1; w < 4; w++) r = threadIdx.x < 3 ? fminf(r, ws[w][threadIdx.x]) : fmaxf(r, ws[w][threadIdx.x]);
+        part[6 * blockIdx.x + threadIdx.x] = r;
+    }
+}
+
+__global__ void k_bbox_final(const float *__restrict__ part, int nb, float *__restrict__ out) {
+    const int k = threadIdx.x;
+    if (k >= 6) return;
+    float r = part[k];
+    for (int b = 1; b < nb; b++) r = k < 3 ? fminf(r, part[6 * b + k]) : fmaxf(r, part[6 * b + k]);
+    out[k] = r;
 }
 
 // exclusive scan of n ints into out[0..n] by one 1024-thread block (tiles of 1024)
@@ -610,9 +714,9 @@ int dmph_nn_gemm(const uint16_t *X, int ldx, const uint16_t *W, const float *bia
     if ((K != 32 && K != 64 && K != 128 && K != 256) || N % 128 || (kstep != 16 && kstep != 32)) return -1;
     const dim3 grid((M + kRows - 1) / kRows, N / kCols);
     const size_t lds = (size_t)kCols * K * sizeof(bf16_t);   // <= 64 KB
-    if (kstep == 32)
+    if (kstep == 32) {
         hipLaunchKernelGGL(k_gemm<32>, grid, dim3(256), lds, (hipStream_t)stream, X, ldx, W, bias, R, ldr, Y, ldy, M, K, relu);
-    else
+    } else
         hipLaunchKernelGGL(k_gemm<16>, grid, dim3(256), lds, (hipStream_t)stream, X, ldx, W, bias, R, ldr, Y, ldy, M, K, relu);
     return launch_ok();
 }
@@ -645,10 +749,11 @@ int dmph_nn_gemm_dw(const uint16_t *dY, int lddy, const uint16_t *X, int ldx, fl
 }
 
 int dmph_nn_edge_fwd(const uint16_t *AB, const float *pos, const int *rp, const int *ci, const float *Wc,
-                     const float *bc, uint16_t *S, int N, float radius, void *stream) {
+                     const float *bc, uint16_t *S, int lds, int N, float radius, void *stream) {
     if (N <= 0) return 0;
-    hipLaunchKernelGGL(k_edge_fwd, dim3((N + 3) / 4), dim3(256), 0, (hipStream_t)stream, AB, pos, rp, ci, Wc, bc, S, N,
-                       1.f / radius);
+    if (lds < 128 || lds % 2) return -1;
+    hipLaunchKernelGGL(k_edge_fwd, dim3((N + 3) / 4), dim3(256), 0, (hipStream_t)stream, AB, pos, rp, ci, Wc, bc, S, lds,
+                       N, 1.f / radius);
     return launch_ok();
 }
 
@@ -667,22 +772,48 @@ int dmph_nn_edge_bwd(const uint16_t *AB, const float *pos, const int *rp, const 
     return launch_ok();
 }
 
+static void scan_excl(const int *in, int *out, int n, int *scratch, hipStream_t s) {
+    const int nt = (n + kTile - 1) / kTile;
+    if (nt <= 1) {
+        hipLaunchKernelGGL(k_scan1, dim3(1), dim3(1024), 0, s, in, out, n);
+        return;
+    }
+    hipLaunchKernelGGL(k_scan_tile_sums, dim3(nt), dim3(256), 0, s, in, n, scratch);
+    hipLaunchKernelGGL(k_scan1, dim3(1), dim3(1024), 0, s, scratch, scratch + nt + 1, nt);   // carries, total at [nt]
+    hipLaunchKernelGGL(k_scan_tiles, dim3(nt), dim3(256), 0, s, in, out, n, scratch + nt + 1);
+}
+
+size_t dmph_nn_graph_scratch(int N, int ncells_total) {
+    const int n = N > ncells_total ? N : ncells_total;
+    return (size_t)(2 * ((n + kTile - 1) / kTile) + 2) * sizeof(int);
+}
+
+size_t dmph_nn_bbox_scratch(int N) { (void)N; return 256 * 6 * sizeof(float); }
+
+int dmph_nn_bbox(const float *pos, int N, float *out, float *scratch, void *stream) {
+    if (N <= 0) return -1;
+    const int nb = (N + 255) / 256 < 256 ? (N + 255) / 256 : 256;
+    hipLaunchKernelGGL(k_bbox_part, dim3(nb), dim3(256), 0, (hipStream_t)stream, pos, N, scratch);
+    hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(64), 0, (hipStream_t)stream, scratch, nb, out);
+    return launch_ok();
+}
+
 int dmph_nn_graph(const float *pos, const int *scene, int N, const NnGrid *grid, float radius, int *key, int *counts,
                   int *start, int *fill, int *order, int *sorted, int *deg, int *rp, int *ci, int cap, int *nnz_out,
-                  void *stream) {
+                  int *scratch, void *stream) {
     hipStream_t s = (hipStream_t)stream;
     const int nk = grid->ncell * (grid->nscene > 0 ? grid->nscene : 1);
     if (hipMemsetAsync(counts, 0, (size_t)(nk + 1) * sizeof(int), s) != hipSuccess) return -5;
     if (hipMemsetAsync(fill, 0, (size_t)(nk + 1) * sizeof(int), s) != hipSuccess) return -5;
     const int b = (N + 255) / 256;
     hipLaunchKernelGGL(k_g_keys, dim3(b), dim3(256), 0, s, pos, scene, N, *grid, key, counts);
-    hipLaunchKernelGGL(k_scan1, dim3(1), dim3(1024), 0, s, counts, start, nk);
+    scan_excl(counts, start, nk, scratch, s);
     hipLaunchKernelGGL(k_g_scatter, dim3(b), dim3(256), 0, s, key, start, fill, order, N);
     hipLaunchKernelGGL(k_g_rank, dim3(b), dim3(256), 0, s, key, start, order, sorted, N);
     const float r2 = radius * radius;
     hipLaunchKernelGGL(k_g_rows<false>, dim3(b), dim3(256), 0, s, pos, key, start, sorted, N, *grid, r2, deg, nullptr,
                        nullptr);
-    hipLaunchKernelGGL(k_scan1, dim3(1), dim3(1024), 0, s, deg, rp, N);
+    scan_excl(deg, rp, N, scratch, s);
     int nnz = 0;
     if (hipMemcpyAsync(&nnz, rp + N, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess) return -5;
     if (hipStreamSynchronize(s) != hipSuccess) return -5;

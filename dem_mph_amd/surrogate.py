@@ -63,7 +63,8 @@ def nnlib():
         from . import share_hip_runtime
         share_hip_runtime()
         _nn = C.CDLL(NN_LIB)
-        for f in ("dmph_nn_dw_scratch", "dmph_nn_edge_scratch", "dmph_nn_head_scratch", "dmph_nn_mse_scratch"):
+        for f in ("dmph_nn_dw_scratch", "dmph_nn_edge_scratch", "dmph_nn_head_scratch", "dmph_nn_mse_scratch",
+                  "dmph_nn_graph_scratch", "dmph_nn_bbox_scratch"):
             getattr(_nn, f).restype = C.c_size_t
     return _nn
 
@@ -141,6 +142,8 @@ class Surrogate:
         self.stream = C.c_void_p(torch.cuda.current_stream(self.dev).cuda_stream)
         self._ws = None
         self._graph_ws = None
+        self._bbox = None
+        self._folded = None   # inference: per round [n1 left | n1 right @ w2] (rebuilt after a weight change)
 
     # ------------------------------------------------------------------ parameters
     def _refresh_bf16(self) -> None:
@@ -183,14 +186,21 @@ class Surrogate:
             if l.b is not None:
                 l.b.copy_(self.torch.tensor(sd[name + ".b"], dtype=self.torch.float32))
         self._refresh_bf16()
+        self._folded = None
 
     # ------------------------------------------------------------------ graph
     def graph(self, pos, scene=None, nscene: int = 1) -> Tuple[object, object, int]:
         """CSR radius graph (r <= 3 d) of the nodes at pos [N, 3] (FP32, device)."""
         torch = self.torch
         N = pos.shape[0]
-        lo = pos.min(dim=0).values.detach().cpu().numpy().astype(np.float64) - 1e-6
-        hi = pos.max(dim=0).values.detach().cpu().numpy().astype(np.float64) + 1e-6
+        lib = nnlib()
+        stream = C.c_void_p(torch.cuda.current_stream(self.dev).cuda_stream)
+        if self._bbox is None:
+            self._bbox = (torch.empty(6, dtype=torch.float32, device=self.dev),
+                          torch.empty(lib.dmph_nn_bbox_scratch(N) // 4 + 1, dtype=torch.float32, device=self.dev))
+        _chk(lib.dmph_nn_bbox(_p(pos), N, _p(self._bbox[0]), _p(self._bbox[1]), stream), "dmph_nn_bbox")
+        box = self._bbox[0].cpu().numpy().astype(np.float64)
+        lo, hi = box[:3] - 1e-6, box[3:] + 1e-6
         w = self.radius * (1.0 + 1e-5)            # cubic cells at least the radius wide
         cc = np.maximum(np.ceil((hi - lo) / w).astype(int), 1)
         g = NnGrid()
@@ -205,17 +215,17 @@ class Surrogate:
         if ws is None or ws["N"] < N or ws["nk"] < nk:
             i32 = lambda n: torch.empty(max(n, 1), dtype=torch.int32, device=self.dev)
             ws = {"N": N, "nk": nk, "key": i32(N), "order": i32(N), "sorted": i32(N), "deg": i32(N),
-                  "counts": i32(nk + 1), "start": i32(nk + 1), "fill": i32(nk + 1), "cap": 0, "ci": None}
+                  "counts": i32(nk + 1), "start": i32(nk + 1), "fill": i32(nk + 1), "cap": 0, "ci": None,
+                  "scan": i32(lib.dmph_nn_graph_scratch(N, nk) // 4 + 1)}
             self._graph_ws = ws
         rp = torch.empty(N + 1, dtype=torch.int32, device=self.dev)
         nnz = C.c_int(0)
-        lib = nnlib()
-        stream = C.c_void_p(torch.cuda.current_stream(self.dev).cuda_stream)
         for _ in range(2):
             ci = ws["ci"]
             rc = lib.dmph_nn_graph(_p(pos), _p(scene), N, C.byref(g), C.c_float(self.radius), _p(ws["key"]),
                                    _p(ws["counts"]), _p(ws["start"]), _p(ws["fill"]), _p(ws["order"]),
-                                   _p(ws["sorted"]), _p(ws["deg"]), _p(rp), _p(ci), ws["cap"], C.byref(nnz), stream)
+                                   _p(ws["sorted"]), _p(ws["deg"]), _p(rp), _p(ci), ws["cap"], C.byref(nnz),
+                                   _p(ws["scan"]), stream)
             if rc == -2:
                 ws["cap"] = int(nnz.value * 1.25) + 1024
                 ws["ci"] = torch.empty(ws["cap"], dtype=torch.int32, device=self.dev)
@@ -283,9 +293,33 @@ class Surrogate:
         _chk(nnlib().dmph_nn_gemm_dw(_p(dY), lddy, _p(X), ldx, _p(layer.gw), _p(layer.gb), M, layer.n_in, layer.n_out,
                                      ws["groups"], _p(ws["scratch"]), 0, self._s()), "gemm_dw")
 
+    def _fold(self):
+        """Inference weights: the aggregation's linear layer folded into the node
+        MLP's first layer, Wn1 [h | W2 S] = [Wn1_h | Wn1_a W2] [h | S] (FP32
+        product, then the bf16 copy), so inference skips one GEMM per round."""
+        if self._folded is None:
+            lib = nnlib()
+            W = WIDTH
+            rng = np.random.default_rng(0)
+            out = []
+            for L in self.rounds:
+                f = Layer(self.torch, self.dev, W, 2 * W, True, 0.0, rng)
+                n1, w2 = L["n1"].w.detach().cpu().numpy().astype(np.float64), L["w2"].w.detach().cpu().numpy().astype(np.float64)
+                fw = np.concatenate([n1[:, :W], n1[:, W:] @ w2], axis=1)
+                f.w.copy_(self.torch.tensor(fw, dtype=self.torch.float32))
+                f.b.copy_(L["n1"].b)
+                z = self.torch.zeros_like(f.w)
+                _chk(lib.dmph_nn_adam(_p(f.w), _p(z), _p(self.torch.zeros_like(f.w)), _p(self.torch.zeros_like(f.w)),
+                                      f.w.numel(), C.c_float(0.0), C.c_float(0.9), C.c_float(0.999), C.c_float(1.0), 1,
+                                      C.c_float(0.0), _p(f.wb), _p(f.wbt), f.n_out, f.n_in, self._s()), "bf16 copy")
+                out.append(f)
+            self._folded = out
+        return self._folded
+
     def forward(self, x0, pos, rp, ci, N: int, train: bool = True):
         """Accelerations (scaled) y [N, 3] FP32; with `train` every round's
-        activations are kept for backward, else two slots alternate."""
+        activations are kept for backward, else two slots alternate and the
+        aggregation's linear layer is folded into the node MLP (_fold)."""
         lib = nnlib()
         ws = self._workspace(N, train)
         W = WIDTH
@@ -294,15 +328,21 @@ class Surrogate:
         hslot = (lambda r: r) if train else (lambda r: r % 2)
         self._gemm(x0, FEAT, self.enc1, ws["h1"], W, N, True)
         self._gemm(ws["h1"], W, self.enc2, ws["hin"][0], 2 * W, N, False)
+        folded = None if train else self._fold()
         for r, L in enumerate(self.rounds):
             hin = ws["hin"][hslot(r)]
             ab, sr, u = ws["ab"][slot(r)], ws["s"][slot(r)], ws["u"][slot(r)]
             self._gemm(hin, 2 * W, L["ab"], ab, 2 * W, N, False)
-            _chk(lib.dmph_nn_edge_fwd(_p(ab), _p(pos), _p(rp), _p(ci), _p(L["c"].w), _p(L["c"].b),
-                                      _p(sr), N, C.c_float(self.radius), s), "edge_fwd")
-            # agg into the right half of hin_r
-            self._gemm(sr, W, L["w2"], hin[:, W:], 2 * W, N, False)
-            self._gemm(hin, 2 * W, L["n1"], u, W, N, True)
+            if train:
+                _chk(lib.dmph_nn_edge_fwd(_p(ab), _p(pos), _p(rp), _p(ci), _p(L["c"].w), _p(L["c"].b),
+                                          _p(sr), W, N, C.c_float(self.radius), s), "edge_fwd")
+                # agg into the right half of hin_r
+                self._gemm(sr, W, L["w2"], hin[:, W:], 2 * W, N, False)
+                self._gemm(hin, 2 * W, L["n1"], u, W, N, True)
+            else:   # S straight into the right half of hin_r, then the folded node layer
+                _chk(lib.dmph_nn_edge_fwd(_p(ab), _p(pos), _p(rp), _p(ci), _p(L["c"].w), _p(L["c"].b),
+                                          _p(hin[:, W:]), 2 * W, N, C.c_float(self.radius), s), "edge_fwd")
+                self._gemm(hin, 2 * W, folded[r], u, W, N, True)
             self._gemm(u, W, L["n2"], ws["hin"][hslot(r + 1)], 2 * W, N, False, R=hin, ldr=2 * W)
         self._gemm(ws["hin"][hslot(ROUNDS)], 2 * W, self.dec1, ws["g"], W, N, True)
         _chk(lib.dmph_nn_head(_p(ws["g"]), _p(self.dec2.w), _p(self.dec2.b), _p(ws["y"]), N, s), "head")
@@ -355,6 +395,7 @@ class Surrogate:
 
     def adam(self, lr: float, b1: float = 0.9, b2: float = 0.999, eps: float = 1e-8, wd: float = 0.0) -> None:
         self.adam_step += 1
+        self._folded = None
         lib = nnlib()
         s = self._s()
         for l in self.layers:

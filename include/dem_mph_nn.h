@@ -47,9 +47,11 @@ int dmph_nn_gemm_dw(const uint16_t *dY, int lddy, const uint16_t *X, int ldx, fl
                     int groups, float *scratch, int accumulate, void *stream);
 
 /* S_i = sum_{j in row i} relu(A_i + B_j + Wc eps_ij + bc); AB = [A | B] as [N][256],
- * eps_ij = ((x_j - x_i) / radius, |x_j - x_i| / radius), Wc [128][4], bc [128], S [N][128] */
+ * eps_ij = ((x_j - x_i) / radius, |x_j - x_i| / radius), Wc [128][4], bc [128],
+ * S row i at S + i * lds (128 values; lds >= 128, even: lets S land in the right
+ * half of a [h | S] buffer) */
 int dmph_nn_edge_fwd(const uint16_t *AB, const float *pos, const int *rp, const int *ci, const float *Wc,
-                     const float *bc, uint16_t *S, int N, float radius, void *stream);
+                     const float *bc, uint16_t *S, int lds, int N, float radius, void *stream);
 /* gradients of edge_fwd given dS [N][128] FP32: dAB [N][256] bf16, dWc, dbc (+)=;
  * the graph must be symmetric; scratch of dmph_nn_edge_scratch bytes */
 size_t dmph_nn_edge_scratch(int N);
@@ -59,10 +61,17 @@ int dmph_nn_edge_bwd(const uint16_t *AB, const float *pos, const int *rp, const 
 
 /* radius graph (CSR rp[N + 1], ci[cap]) of the nodes at pos [N][3] (scene ids or NULL):
  * rows ascending by cell then node; *nnz_out = edges; returns -2 if cap < nnz (call again).
- * Work arrays: key, order, sorted, deg [N]; counts, start, fill [cells * scenes + 1]. */
+ * Work arrays: key, order, sorted, deg [N]; counts, start, fill [cells * scenes + 1];
+ * scratch of dmph_nn_graph_scratch(N, cells * scenes) bytes (scan carries). */
+size_t dmph_nn_graph_scratch(int N, int ncells_total);
 int dmph_nn_graph(const float *pos, const int *scene, int N, const NnGrid *grid, float radius, int *key, int *counts,
                   int *start, int *fill, int *order, int *sorted, int *deg, int *rp, int *ci, int cap, int *nnz_out,
-                  void *stream);
+                  int *scratch, void *stream);
+
+/* bounding box of pos [N][3] into out[6] (min x y z, max x y z; device memory);
+ * scratch of dmph_nn_bbox_scratch(N) bytes */
+size_t dmph_nn_bbox_scratch(int N);
+int dmph_nn_bbox(const float *pos, int N, float *out, float *scratch, void *stream);
 
 /* node features X [N][32] bf16 from vel_hist [5][N][3] (newest first), class 0/1/2, positions */
 int dmph_nn_features(const float *vel_hist, int N, const int *cls, const float *pos, const NnFeat *ft, uint16_t *X,

@@ -21,6 +21,17 @@ def _declared():
     return sorted(set(re.findall(r"\b(dmph_[a-z_]+)\s*\(", txt)))
 
 
+def test_network_library_exports_every_declared_symbol():
+    """libdemmph_nn.so exports each function include/dem_mph_nn.h declares."""
+    from dem_mph_amd import surrogate as SG
+    txt = open(os.path.join(ROOT, "include", "dem_mph_nn.h")).read()
+    names = sorted(set(re.findall(r"\b(dmph_nn_[a-z0-9_]+)\s*\(", txt)))
+    assert len(names) >= 15
+    lib = SG.nnlib()
+    for n in names:
+        assert hasattr(lib, n), n
+
+
 def test_library_exports_every_declared_symbol():
     lib = D.lib()
     names = _declared()

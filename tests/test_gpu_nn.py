@@ -149,6 +149,28 @@ def test_network_gradients_match_fp64_oracle(S):
     assert not bad, bad
 
 
+def test_inference_forward_matches_fp64_oracle(S):
+    """Inference folds the aggregation's linear layer into the node layer (one
+    GEMM fewer per round): outputs within 5% (norm) of the FP64 oracle, also
+    after an Adam step has changed the weights the fold was built from."""
+    sur, sc, pos, vh, cls, tgt, mask = _small_problem(S, seed=2)
+    n = sc.n
+    rp, ci, nnz = sur.graph(pos)
+    x0 = sur.features(vh, cls, pos)
+    count = int(mask.sum().item())
+    X0 = _host_bf16(x0[:n])
+    for it in range(2):
+        y = sur.forward(x0, pos, rp, ci, n, train=False).clone()
+        yr, _ = NO.forward(sur.state_dict(), X0, pos.cpu().numpy().astype(np.float64), rp.cpu().numpy(),
+                           ci.cpu().numpy()[:nnz], sur.radius)
+        ey = np.linalg.norm(y.cpu().numpy() - yr) / np.linalg.norm(yr)
+        assert ey < 0.05, (it, ey)
+        if it == 0:
+            sur.forward(x0, pos, rp, ci, n)
+            sur.loss_and_backward(x0, pos, rp, ci, n, tgt, mask, count)
+            sur.adam(3e-2)
+
+
 def test_training_reduces_loss(S):
     """Adam on a target the inputs determine (twice the newest velocity): the
     loss falls at least tenfold in 150 steps."""
