@@ -47,7 +47,9 @@ typedef float f4_t __attribute__((ext_vector_type(4)));
 // KS: reduction step of the instruction, 16 (v_mfma_f32_16x16x16_bf16) or 32
 // (v_mfma_f32_16x16x32_bf16, gfx950).  Lane l of a 16x16xKS tile holds A row
 // l % 16, k = (KS/4) * (l / 16) + [0, KS/4); B column l % 16, same k; D column
-// l % 16, rows 4 * (l / 16) + [0, 4).
+// l % 16, rows 4 * (l / 16) + [0, 4).  The tiles are computed transposed
+// (A = W, B = X: D rows are output columns), so a lane ends with 4 adjacent
+// outputs of one row and stores them as one 8-byte word.
 // ---------------------------------------------------------------------------------------------
 
 constexpr int kRows = 128;
@@ -64,7 +66,7 @@ __device__ __forceinline__ int wl_at(int r, int k, int K) {
 template <int KS>
 __global__ void __launch_bounds__(256) k_gemm(const bf16_t *__restrict__ X, int ldx, const bf16_t *__restrict__ W,
                                               const float *__restrict__ bias, const bf16_t *__restrict__ R, int ldr,
-                                              bf16_t *__restrict__ Y, int ldy, int M, int K, int relu) {
+                                              bf16_t *__restrict__ Y, int ldy, int M, int K, int flags) {
     extern __shared__ bf16_t wl[];   // [128][K], 16-byte granules XOR-swizzled by row (wl_at)
     const int n0 = blockIdx.y * kCols;
     for (int t = threadIdx.x; t < kCols * K / 8; t += blockDim.x) {   // 16-byte copies
@@ -99,7 +101,7 @@ __global__ void __launch_bounds__(256) k_gemm(const bf16_t *__restrict__ X, int 
                 const uint2 v = *reinterpret_cast<const uint2 *>(wl + wl_at(c * 16 + lr, k + lk, K));
                 const s4_t bf = s4_t{(short)(v.x & 0xffff), (short)(v.x >> 16), (short)(v.y & 0xffff), (short)(v.y >> 16)};
 #pragma unroll
-                for (int a = 0; a < 2; a++) acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af[a], bf, acc[a][c], 0, 0, 0);
+                for (int a = 0; a < 2; a++) acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(bf, af[a], acc[a][c], 0, 0, 0);
             }
         } else {
             b8_t af[2];
@@ -115,26 +117,35 @@ __global__ void __launch_bounds__(256) k_gemm(const bf16_t *__restrict__ X, int 
                 const uint4 v = *reinterpret_cast<const uint4 *>(wl + wl_at(c * 16 + lr, k + lk, K));
                 const b8_t bf = __builtin_bit_cast(b8_t, v);
 #pragma unroll
-                for (int a = 0; a < 2; a++) acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[a], bf, acc[a][c], 0, 0, 0);
+                for (int a = 0; a < 2; a++) acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bf, af[a], acc[a][c], 0, 0, 0);
             }
         }
     }
-    // epilogue: bias, residual, ReLU, bf16 store
+    // epilogue: bias, residual, ReLU; acc[a][c][e] is row row0 + 16a + lr, column
+    // n0 + 16c + 4 (lane / 16) + e
+    const bool relu = flags & DMPH_NN_RELU, half = flags & DMPH_NN_F16_OUT;
 #pragma unroll
     for (int c = 0; c < 8; c++) {
-        const int col = n0 + c * 16 + lr;
-        const float b = bias ? bias[col] : 0.f;
+        const int col = n0 + c * 16 + (lane >> 4) * 4;
+        const float4 b = bias ? *reinterpret_cast<const float4 *>(bias + col) : make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
-        for (int a = 0; a < 2; a++)
-#pragma unroll
-            for (int e = 0; e < 4; e++) {
-                const int r = row0 + a * 16 + (lane >> 4) * 4 + e;
-                if (r >= M) continue;
-                float v = acc[a][c][e] + b;
-                if (R) v += bf2f(R[(size_t)r * ldr + col]);
-                if (relu) v = fmaxf(v, 0.f);
-                Y[(size_t)r * ldy + col] = f2bf(v);
+        for (int a = 0; a < 2; a++) {
+            const int r = row0 + a * 16 + lr;
+            if (r >= M) continue;
+            float v[4] = {acc[a][c][0] + b.x, acc[a][c][1] + b.y, acc[a][c][2] + b.z, acc[a][c][3] + b.w};
+            if (R) {
+                const uint2 q = *reinterpret_cast<const uint2 *>(R + (size_t)r * ldr + col);
+                v[0] += bf2f((bf16_t)(q.x & 0xffff)); v[1] += bf2f((bf16_t)(q.x >> 16));
+                v[2] += bf2f((bf16_t)(q.y & 0xffff)); v[3] += bf2f((bf16_t)(q.y >> 16));
             }
+            if (relu)
+#pragma unroll
+                for (int e = 0; e < 4; e++) v[e] = fmaxf(v[e], 0.f);
+            uint32_t o[4];
+#pragma unroll
+            for (int e = 0; e < 4; e++) o[e] = half ? (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)v[e]) : (uint32_t)f2bf(v[e]);
+            *reinterpret_cast<uint2 *>(Y + (size_t)r * ldy + col) = make_uint2(o[0] | o[1] << 16, o[2] | o[3] << 16);
+        }
     }
 }
 
@@ -300,6 +311,81 @@ __global__ void __launch_bounds__(256) k_edge_fwd(const bf16_t *__restrict__ AB,
             }
         }
     }
+    *reinterpret_cast<uint32_t *>(S + (size_t)i * lds + f) = (uint32_t)f2bf(s0) | ((uint32_t)f2bf(s1) << 16);
+}
+
+typedef _Float16 h2_t __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ uint32_t h2bits(h2_t v) { return __builtin_bit_cast(uint32_t, v); }
+__device__ __forceinline__ h2_t bitsh2(uint32_t v) { return __builtin_bit_cast(h2_t, v); }
+
+// one wave per node, lanes over the row's edges: eps[2q] = (dx, dy) / r, eps[2q + 1] = (dz, |d|) / r
+__global__ void __launch_bounds__(256) k_edge_eps(const float *__restrict__ pos, const int *__restrict__ rp,
+                                                  const int *__restrict__ ci, uint2 *__restrict__ eps, int N,
+                                                  float inv_r) {
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (i >= N) return;
+    const float xi = pos[3 * i], yi = pos[3 * i + 1], zi = pos[3 * i + 2];
+    for (int q = rp[i] + (threadIdx.x & 63); q < rp[i + 1]; q += 64) {
+        const int j = ci[q];
+        const float dx = pos[3 * j] - xi, dy = pos[3 * j + 1] - yi, dz = pos[3 * j + 2] - zi;
+        const float d = sqrtf(dx * dx + dy * dy + dz * dz);
+        eps[q] = make_uint2(h2bits(h2_t{(_Float16)(dx * inv_r), (_Float16)(dy * inv_r)}),
+                            h2bits(h2_t{(_Float16)(dz * inv_r), (_Float16)(d * inv_r)}));
+    }
+}
+
+// Inference edge sums in packed IEEE half: lane l holds features 2l, 2l + 1 as
+// one pair; per edge c = (A_i + bc) + B_j + sum_k Wc[:, k] eps_k is 1 add and 4
+// fused multiply-adds on pairs, then max and add into a half partial that is
+// added to the FP32 sums every 16 edges.  The node index, row bounds, column
+// indices and edge features are wave-uniform (scalar loads); B_j is one 4-byte
+// load per lane.
+__global__ void __launch_bounds__(256) k_edge_fwd_h(const h2_t *__restrict__ AB, const int *__restrict__ rp,
+                                                    const int *__restrict__ ci, const uint2 *__restrict__ eps,
+                                                    const float *__restrict__ Wc, const float *__restrict__ bc,
+                                                    bf16_t *__restrict__ S, int lds, int N) {
+    const int i = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
+    if (i >= N) return;
+    const int lane = threadIdx.x & 63, f = 2 * lane;
+    const float4 w0 = *reinterpret_cast<const float4 *>(Wc + 4 * f), w1 = *reinterpret_cast<const float4 *>(Wc + 4 * f + 4);
+    const h2_t wx = {(_Float16)w0.x, (_Float16)w1.x}, wy = {(_Float16)w0.y, (_Float16)w1.y};
+    const h2_t wz = {(_Float16)w0.z, (_Float16)w1.z}, wd = {(_Float16)w0.w, (_Float16)w1.w};
+    const h2_t ai = AB[(size_t)i * 128 + lane];
+    const h2_t a2 = {(_Float16)((float)ai.x + bc[f]), (_Float16)((float)ai.y + bc[f + 1])};
+    const h2_t zero = {(_Float16)0.f, (_Float16)0.f};
+    const uint32_t lane_bytes = 4u * lane;
+    // B_j pair of this lane: uniform row address + lane offset (one scalar base per load)
+    auto b_of = [&](int j) {
+        const char *row = reinterpret_cast<const char *>(AB) + (size_t)j * 512 + 256;
+        return *reinterpret_cast<const h2_t *>(row + lane_bytes);
+    };
+    float s0 = 0.f, s1 = 0.f;
+    const int q1 = rp[i + 1];
+    int q = rp[i];
+    auto edge = [&](h2_t bj, uint2 e, h2_t acc) {
+        const h2_t exy = bitsh2(e.x), ezd = bitsh2(e.y);
+        h2_t c = a2 + bj;
+        c = __builtin_elementwise_fma(wx, __builtin_shufflevector(exy, exy, 0, 0), c);
+        c = __builtin_elementwise_fma(wy, __builtin_shufflevector(exy, exy, 1, 1), c);
+        c = __builtin_elementwise_fma(wz, __builtin_shufflevector(ezd, ezd, 0, 0), c);
+        c = __builtin_elementwise_fma(wd, __builtin_shufflevector(ezd, ezd, 1, 1), c);
+        return acc + __builtin_elementwise_maximum(c, zero);   // NaN-propagating: an overflow stays visible
+    };
+    for (; q + kEdgeBatch <= q1; q += kEdgeBatch) {
+        h2_t bj[kEdgeBatch];
+#pragma unroll
+        for (int t = 0; t < kEdgeBatch; t++) bj[t] = b_of(ci[q + t]);
+        h2_t acc = zero;
+#pragma unroll
+        for (int t = 0; t < kEdgeBatch; t++) acc = edge(bj[t], eps[q + t], acc);
+        s0 += (float)acc.x;
+        s1 += (float)acc.y;
+    }
+    h2_t acc = zero;
+    for (; q < q1; q++) acc = edge(b_of(ci[q]), eps[q], acc);
+    s0 += (float)acc.x;
+    s1 += (float)acc.y;
     *reinterpret_cast<uint32_t *>(S + (size_t)i * lds + f) = (uint32_t)f2bf(s0) | ((uint32_t)f2bf(s1) << 16);
 }
 
@@ -712,6 +798,7 @@ int dmph_nn_gemm(const uint16_t *X, int ldx, const uint16_t *W, const float *bia
     if (M <= 0) return 0;
     // K / 8 must be a power of two (the LDS swizzle stays inside a row)
     if ((K != 32 && K != 64 && K != 128 && K != 256) || N % 128 || (kstep != 16 && kstep != 32)) return -1;
+    if (ldy % 4 || (R && ldr % 4) || (relu & ~(DMPH_NN_RELU | DMPH_NN_F16_OUT))) return -1;   // 8-byte row words
     const dim3 grid((M + kRows - 1) / kRows, N / kCols);
     const size_t lds = (size_t)kCols * K * sizeof(bf16_t);   // <= 64 KB
     if (kstep == 32) {
@@ -754,6 +841,23 @@ int dmph_nn_edge_fwd(const uint16_t *AB, const float *pos, const int *rp, const 
     if (lds < 128 || lds % 2) return -1;
     hipLaunchKernelGGL(k_edge_fwd, dim3((N + 3) / 4), dim3(256), 0, (hipStream_t)stream, AB, pos, rp, ci, Wc, bc, S, lds,
                        N, 1.f / radius);
+    return launch_ok();
+}
+
+int dmph_nn_edge_eps(const float *pos, const int *rp, const int *ci, uint32_t *eps, int N, float radius, void *stream) {
+    if (N <= 0) return 0;
+    hipLaunchKernelGGL(k_edge_eps, dim3((N + 3) / 4), dim3(256), 0, (hipStream_t)stream, pos, rp, ci,
+                       reinterpret_cast<uint2 *>(eps), N, 1.f / radius);
+    return launch_ok();
+}
+
+int dmph_nn_edge_fwd_f16(const uint16_t *AB, const int *rp, const int *ci, const uint32_t *eps, const float *Wc,
+                         const float *bc, uint16_t *S, int lds, int N, void *stream) {
+    if (N <= 0) return 0;
+    if (lds < 128 || lds % 2) return -1;
+    hipLaunchKernelGGL(k_edge_fwd_h, dim3((N + 3) / 4), dim3(256), 0, (hipStream_t)stream,
+                       reinterpret_cast<const h2_t *>(AB), rp, ci, reinterpret_cast<const uint2 *>(eps), Wc, bc, S, lds,
+                       N);
     return launch_ok();
 }
 

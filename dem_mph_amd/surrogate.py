@@ -39,6 +39,7 @@ NN_LIB = os.path.join(PKG, "libdemmph_nn.so")
 WIDTH = 128
 ROUNDS = 6
 FEAT = 32
+GEMM_RELU, GEMM_F16_OUT = 1, 2   # dmph_nn_gemm flags (DMPH_NN_RELU, DMPH_NN_F16_OUT)
 HIST = 5
 
 
@@ -280,12 +281,14 @@ class Surrogate:
     def _s(self) -> C.c_void_p:
         return C.c_void_p(self.torch.cuda.current_stream(self.dev).cuda_stream)
 
-    def _gemm(self, X, ldx, layer: Layer, Y, ldy, M, relu, R=None, ldr=0, transposed=False, K=None, Nout=None):
+    def _gemm(self, X, ldx, layer: Layer, Y, ldy, M, relu, R=None, ldr=0, transposed=False, K=None, Nout=None,
+              half=False):
         w = layer.wbt if transposed else layer.wb
         k = K if K is not None else (layer.n_out if transposed else layer.n_in)
         n = Nout if Nout is not None else (layer.n_in if transposed else layer.n_out)
         bias = None if transposed else layer.b
-        _chk(nnlib().dmph_nn_gemm(_p(X), ldx, _p(w), _p(bias), _p(R), ldr, _p(Y), ldy, M, k, n, 1 if relu else 0,
+        flags = (GEMM_RELU if relu else 0) | (GEMM_F16_OUT if half else 0)
+        _chk(nnlib().dmph_nn_gemm(_p(X), ldx, _p(w), _p(bias), _p(R), ldr, _p(Y), ldy, M, k, n, flags,
                                   self.kstep, self._s()), "gemm")
 
     def _dw(self, dY, lddy, X, ldx, layer: Layer, M):
@@ -329,10 +332,15 @@ class Surrogate:
         self._gemm(x0, FEAT, self.enc1, ws["h1"], W, N, True)
         self._gemm(ws["h1"], W, self.enc2, ws["hin"][0], 2 * W, N, False)
         folded = None if train else self._fold()
+        if not train:   # edge features once per graph, for the half-precision edge sums
+            if ws.get("eps") is None or ws["eps"].numel() < 2 * ci.numel():
+                ws["eps"] = self.torch.empty(2 * ci.numel(), dtype=self.torch.int32, device=self.dev)
+            _chk(lib.dmph_nn_edge_eps(_p(pos), _p(rp), _p(ci), _p(ws["eps"]), N, C.c_float(self.radius), s),
+                 "edge_eps")
         for r, L in enumerate(self.rounds):
             hin = ws["hin"][hslot(r)]
             ab, sr, u = ws["ab"][slot(r)], ws["s"][slot(r)], ws["u"][slot(r)]
-            self._gemm(hin, 2 * W, L["ab"], ab, 2 * W, N, False)
+            self._gemm(hin, 2 * W, L["ab"], ab, 2 * W, N, False, half=not train)
             if train:
                 _chk(lib.dmph_nn_edge_fwd(_p(ab), _p(pos), _p(rp), _p(ci), _p(L["c"].w), _p(L["c"].b),
                                           _p(sr), W, N, C.c_float(self.radius), s), "edge_fwd")
@@ -340,8 +348,8 @@ class Surrogate:
                 self._gemm(sr, W, L["w2"], hin[:, W:], 2 * W, N, False)
                 self._gemm(hin, 2 * W, L["n1"], u, W, N, True)
             else:   # S straight into the right half of hin_r, then the folded node layer
-                _chk(lib.dmph_nn_edge_fwd(_p(ab), _p(pos), _p(rp), _p(ci), _p(L["c"].w), _p(L["c"].b),
-                                          _p(hin[:, W:]), 2 * W, N, C.c_float(self.radius), s), "edge_fwd")
+                _chk(lib.dmph_nn_edge_fwd_f16(_p(ab), _p(rp), _p(ci), _p(ws["eps"]), _p(L["c"].w), _p(L["c"].b),
+                                              _p(hin[:, W:]), 2 * W, N, s), "edge_fwd_f16")
                 self._gemm(hin, 2 * W, folded[r], u, W, N, True)
             self._gemm(u, W, L["n2"], ws["hin"][hslot(r + 1)], 2 * W, N, False, R=hin, ldr=2 * W)
         self._gemm(ws["hin"][hslot(ROUNDS)], 2 * W, self.dec1, ws["g"], W, N, True)

@@ -34,10 +34,15 @@ typedef struct NnFeat {
     float inv_r;
 } NnFeat;
 
+/* gemm flags */
+#define DMPH_NN_RELU 1      /* ReLU after bias and residual */
+#define DMPH_NN_F16_OUT 2   /* Y in IEEE half instead of bf16 */
+
 /* Y[:, 0:N) (row stride ldy) = act(X[M][K] W^T + bias [+ R]); K in {32, 64, 128, 256},
- * N % 128 == 0; relu 0/1; kstep 16 (v_mfma_f32_16x16x16_bf16) or 32 (gfx950 16x16x32) */
+ * N % 128 == 0, ldy and ldr multiples of 4; flags DMPH_NN_*; kstep 16
+ * (v_mfma_f32_16x16x16_bf16) or 32 (gfx950 16x16x32) */
 int dmph_nn_gemm(const uint16_t *X, int ldx, const uint16_t *W, const float *bias, const uint16_t *R, int ldr,
-                 uint16_t *Y, int ldy, int M, int K, int N, int relu, int kstep, void *stream);
+                 uint16_t *Y, int ldy, int M, int K, int N, int flags, int kstep, void *stream);
 
 /* dW[N][K] (+)= dY^T X, db[N] (+)= sum_m dY (db may be NULL); K in {32, 128, 256},
  * N % 128 == 0; `groups` row chunks summed in a fixed order (deterministic);
@@ -52,6 +57,14 @@ int dmph_nn_gemm_dw(const uint16_t *dY, int lddy, const uint16_t *X, int ldx, fl
  * half of a [h | S] buffer) */
 int dmph_nn_edge_fwd(const uint16_t *AB, const float *pos, const int *rp, const int *ci, const float *Wc,
                      const float *bc, uint16_t *S, int lds, int N, float radius, void *stream);
+/* Inference form of edge_fwd.  edge_eps writes each edge's features once per
+ * graph as two IEEE-half pairs, eps[q] = ((dx, dy), (dz, |d|)) / radius (row
+ * order of ci; eps holds nnz entries); edge_fwd_f16 then sums with AB in IEEE
+ * half ([N][256], DMPH_NN_F16_OUT), packed half arithmetic in batches of 16
+ * edges added into FP32 sums; S as in edge_fwd (bf16). */
+int dmph_nn_edge_eps(const float *pos, const int *rp, const int *ci, uint32_t *eps, int N, float radius, void *stream);
+int dmph_nn_edge_fwd_f16(const uint16_t *AB, const int *rp, const int *ci, const uint32_t *eps, const float *Wc,
+                         const float *bc, uint16_t *S, int lds, int N, void *stream);
 /* gradients of edge_fwd given dS [N][128] FP32: dAB [N][256] bf16, dWc, dbc (+)=;
  * the graph must be symmetric; scratch of dmph_nn_edge_scratch bytes */
 size_t dmph_nn_edge_scratch(int N);
