@@ -129,21 +129,8 @@ def errors_vs_oracle(D, sc, params, prec, compensated, o, steps):
 
 
 def copy_probe(mib: int) -> None:
-    """One device-to-device copy of `mib` MiB through the HIP runtime: the copy
-    kernel's FETCH_SIZE / WRITE_SIZE under rocprof calibrate the counters
-    against a known byte count."""
-    import ctypes as C
-    from dem_mph_amd import share_hip_runtime
-    share_hip_runtime()
-    hip = C.CDLL("libamdhip64.so.7")   # by soname: the runtime already loaded, if any
-    n = mib << 20
-    a, b = C.c_void_p(), C.c_void_p()
-    assert hip.hipMalloc(C.byref(a), C.c_size_t(n)) == 0 and hip.hipMalloc(C.byref(b), C.c_size_t(n)) == 0
-    assert hip.hipMemset(a, 1, C.c_size_t(n)) == 0
-    assert hip.hipMemcpy(b, a, C.c_size_t(n), 3) == 0   # hipMemcpyDeviceToDevice
-    assert hip.hipDeviceSynchronize() == 0
-    hip.hipFree(a)
-    hip.hipFree(b)
+    from dem_mph_amd.profiling import copy_probe as probe
+    probe(mib)
     print("bench: copy probe of %d MiB done" % mib, file=sys.stderr, flush=True)
 
 

@@ -19,6 +19,7 @@
 // All entry points are extern "C" with plain device pointers and a stream.
 #include <hip/hip_runtime.h>
 #include <algorithm>
+#include <type_traits>
 #include <stdint.h>
 
 #include "dem_mph_nn.h"
@@ -52,6 +53,14 @@ typedef float f4_t __attribute__((ext_vector_type(4)));
 // outputs of one row and stores them as one 8-byte word.
 // ---------------------------------------------------------------------------------------------
 
+// Workgroups are handed to the 8 XCDs (each with its own L2) round-robin by
+// index.  For gather kernels whose neighbouring nodes share data, this maps the
+// launch's G workgroups so that each XCD takes one contiguous range of them.
+__device__ __forceinline__ int xcd_block(int G) {
+    const int w = blockIdx.x, q = G >> 3, r = G & 7, x = w & 7;
+    return x * q + min(x, r) + (w >> 3);
+}
+
 constexpr int kRows = 128;
 constexpr int kCols = 128;
 
@@ -63,67 +72,68 @@ __device__ __forceinline__ int wl_at(int r, int k, int K) {
     return r * K + (((k >> 3) ^ (r & m)) << 3) + (k & 7);
 }
 
-template <int KS>
+template <int KS, int K>
 __global__ void __launch_bounds__(256) k_gemm(const bf16_t *__restrict__ X, int ldx, const bf16_t *__restrict__ W,
                                               const float *__restrict__ bias, const bf16_t *__restrict__ R, int ldr,
-                                              bf16_t *__restrict__ Y, int ldy, int M, int K, int flags) {
+                                              bf16_t *__restrict__ Y, int ldy, int M, int flags) {
     extern __shared__ bf16_t wl[];   // [128][K], 16-byte granules XOR-swizzled by row (wl_at)
+    constexpr int NK = K / KS;
+    typedef typename std::conditional<KS == 32, uint4, uint2>::type frag_t;
     const int n0 = blockIdx.y * kCols;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int row0 = blockIdx.x * kRows + wave * 32;
+    const int lr = lane & 15, lk = (lane >> 4) * (KS / 4);
+    // every X fragment of the wave's two 16-row blocks, requested before the W
+    // block is staged so the two latencies overlap (rows past M read as zero)
+    frag_t xf[NK][2];
+#pragma unroll
+    for (int a = 0; a < 2; a++) {
+        const int r = row0 + a * 16 + lr;
+#pragma unroll
+        for (int kk = 0; kk < NK; kk++) {
+            xf[kk][a] = frag_t{};
+            if (r < M) xf[kk][a] = *reinterpret_cast<const frag_t *>(X + (size_t)r * ldx + kk * KS + lk);
+        }
+    }
     for (int t = threadIdx.x; t < kCols * K / 8; t += blockDim.x) {   // 16-byte copies
         const int r = t / (K / 8), c = (t % (K / 8)) * 8;
         *reinterpret_cast<uint4 *>(wl + wl_at(r, c, K)) = *reinterpret_cast<const uint4 *>(W + (size_t)(n0 + r) * K + c);
     }
     __syncthreads();
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int row0 = blockIdx.x * kRows + wave * 32;
-    const int lr = lane & 15, lk = (lane >> 4) * (KS / 4);
     f4_t acc[2][8];
 #pragma unroll
     for (int a = 0; a < 2; a++)
 #pragma unroll
         for (int c = 0; c < 8; c++) acc[a][c] = f4_t{0.f, 0.f, 0.f, 0.f};
-    for (int k = 0; k < K; k += KS) {
-        // A fragments of the wave's two 16-row blocks (rows past M read as zero)
-        if constexpr (KS == 16) {
-            s4_t af[2];
+#pragma unroll
+    for (int kk = 0; kk < NK; kk++) {
+        const int k = kk * KS;
+#pragma unroll
+        for (int c = 0; c < 8; c++) {
+            const frag_t v = *reinterpret_cast<const frag_t *>(wl + wl_at(c * 16 + lr, k + lk, K));
 #pragma unroll
             for (int a = 0; a < 2; a++) {
-                const int r = row0 + a * 16 + lr;
-                if (r < M) {
-                    const uint2 v = *reinterpret_cast<const uint2 *>(X + (size_t)r * ldx + k + lk);
-                    af[a] = s4_t{(short)(v.x & 0xffff), (short)(v.x >> 16), (short)(v.y & 0xffff), (short)(v.y >> 16)};
-                } else {
-                    af[a] = s4_t{0, 0, 0, 0};
-                }
-            }
-#pragma unroll
-            for (int c = 0; c < 8; c++) {
-                const uint2 v = *reinterpret_cast<const uint2 *>(wl + wl_at(c * 16 + lr, k + lk, K));
-                const s4_t bf = s4_t{(short)(v.x & 0xffff), (short)(v.x >> 16), (short)(v.y & 0xffff), (short)(v.y >> 16)};
-#pragma unroll
-                for (int a = 0; a < 2; a++) acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(bf, af[a], acc[a][c], 0, 0, 0);
-            }
-        } else {
-            b8_t af[2];
-#pragma unroll
-            for (int a = 0; a < 2; a++) {
-                const int r = row0 + a * 16 + lr;
-                uint4 v = make_uint4(0, 0, 0, 0);
-                if (r < M) v = *reinterpret_cast<const uint4 *>(X + (size_t)r * ldx + k + lk);
-                af[a] = __builtin_bit_cast(b8_t, v);
-            }
-#pragma unroll
-            for (int c = 0; c < 8; c++) {
-                const uint4 v = *reinterpret_cast<const uint4 *>(wl + wl_at(c * 16 + lr, k + lk, K));
-                const b8_t bf = __builtin_bit_cast(b8_t, v);
-#pragma unroll
-                for (int a = 0; a < 2; a++) acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bf, af[a], acc[a][c], 0, 0, 0);
+                if constexpr (KS == 16)
+                    acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(__builtin_bit_cast(s4_t, v),
+                                                                          __builtin_bit_cast(s4_t, xf[kk][a]), acc[a][c], 0, 0, 0);
+                else
+                    acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(b8_t, v),
+                                                                        __builtin_bit_cast(b8_t, xf[kk][a]), acc[a][c], 0, 0, 0);
             }
         }
     }
     // epilogue: bias, residual, ReLU; acc[a][c][e] is row row0 + 16a + lr, column
     // n0 + 16c + 4 (lane / 16) + e
     const bool relu = flags & DMPH_NN_RELU, half = flags & DMPH_NN_F16_OUT;
+    uint2 rr[2][8];   // residual words, all requested before the first is used
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int c = 0; c < 8; c++) {
+            const int r = row0 + a * 16 + lr;
+            rr[a][c] = make_uint2(0, 0);
+            if (R && r < M) rr[a][c] = *reinterpret_cast<const uint2 *>(R + (size_t)r * ldr + n0 + c * 16 + (lane >> 4) * 4);
+        }
 #pragma unroll
     for (int c = 0; c < 8; c++) {
         const int col = n0 + c * 16 + (lane >> 4) * 4;
@@ -134,7 +144,7 @@ __global__ void __launch_bounds__(256) k_gemm(const bf16_t *__restrict__ X, int 
             if (r >= M) continue;
             float v[4] = {acc[a][c][0] + b.x, acc[a][c][1] + b.y, acc[a][c][2] + b.z, acc[a][c][3] + b.w};
             if (R) {
-                const uint2 q = *reinterpret_cast<const uint2 *>(R + (size_t)r * ldr + col);
+                const uint2 q = rr[a][c];
                 v[0] += bf2f((bf16_t)(q.x & 0xffff)); v[1] += bf2f((bf16_t)(q.x >> 16));
                 v[2] += bf2f((bf16_t)(q.y & 0xffff)); v[3] += bf2f((bf16_t)(q.y >> 16));
             }
@@ -274,7 +284,7 @@ __global__ void __launch_bounds__(256) k_edge_fwd(const bf16_t *__restrict__ AB,
                                                   const int *__restrict__ rp, const int *__restrict__ ci,
                                                   const float *__restrict__ Wc, const float *__restrict__ bc,
                                                   bf16_t *__restrict__ S, int lds, int N, float inv_r) {
-    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int i = xcd_block(gridDim.x) * 4 + (threadIdx.x >> 6);
     if (i >= N) return;
     const int lane = threadIdx.x & 63, f = 2 * lane;
     const float a0 = bf2f(AB[(size_t)i * 256 + f]), a1 = bf2f(AB[(size_t)i * 256 + f + 1]);
@@ -323,7 +333,7 @@ __device__ __forceinline__ h2_t bitsh2(uint32_t v) { return __builtin_bit_cast(h
 __global__ void __launch_bounds__(256) k_edge_eps(const float *__restrict__ pos, const int *__restrict__ rp,
                                                   const int *__restrict__ ci, uint2 *__restrict__ eps, int N,
                                                   float inv_r) {
-    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int i = xcd_block(gridDim.x) * 4 + (threadIdx.x >> 6);
     if (i >= N) return;
     const float xi = pos[3 * i], yi = pos[3 * i + 1], zi = pos[3 * i + 2];
     for (int q = rp[i] + (threadIdx.x & 63); q < rp[i + 1]; q += 64) {
@@ -345,7 +355,7 @@ __global__ void __launch_bounds__(256) k_edge_fwd_h(const h2_t *__restrict__ AB,
                                                     const int *__restrict__ ci, const uint2 *__restrict__ eps,
                                                     const float *__restrict__ Wc, const float *__restrict__ bc,
                                                     bf16_t *__restrict__ S, int lds, int N) {
-    const int i = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
+    const int i = __builtin_amdgcn_readfirstlane(xcd_block(gridDim.x) * 4 + (threadIdx.x >> 6));
     if (i >= N) return;
     const int lane = threadIdx.x & 63, f = 2 * lane;
     const float4 w0 = *reinterpret_cast<const float4 *>(Wc + 4 * f), w1 = *reinterpret_cast<const float4 *>(Wc + 4 * f + 4);
@@ -355,7 +365,8 @@ __global__ void __launch_bounds__(256) k_edge_fwd_h(const h2_t *__restrict__ AB,
     const h2_t a2 = {(_Float16)((float)ai.x + bc[f]), (_Float16)((float)ai.y + bc[f + 1])};
     const h2_t zero = {(_Float16)0.f, (_Float16)0.f};
     const uint32_t lane_bytes = 4u * lane;
-    // B_j pair of this lane: uniform row address + lane offset (one scalar base per load)
+    // B_j pair of this lane: uniform row address + lane offset (a buffer-descriptor
+    // load, whose address needs no vector op, measured 17 % slower here)
     auto b_of = [&](int j) {
         const char *row = reinterpret_cast<const char *>(AB) + (size_t)j * 512 + 256;
         return *reinterpret_cast<const h2_t *>(row + lane_bytes);
@@ -398,7 +409,7 @@ __global__ void __launch_bounds__(256) k_edge_bwd(const bf16_t *__restrict__ AB,
                                                   float *__restrict__ Pw, int N, float inv_r) {
     __shared__ float red[4][128 * 5];
     const int wave = threadIdx.x >> 6;
-    const int n = blockIdx.x * 4 + wave;
+    const int n = xcd_block(gridDim.x) * 4 + wave;
     const int f = 2 * (threadIdx.x & 63);
     const float4 w0 = *reinterpret_cast<const float4 *>(Wc + 4 * f), w1 = *reinterpret_cast<const float4 *>(Wc + 4 * f + 4);
     const float b0 = bc[f], b1 = bc[f + 1];
@@ -585,7 +596,7 @@ __global__ void k_g_rank(const int *key, const int *start, const int *order, int
 template <bool FILL>
 __global__ void k_g_rows(const float *pos, const int *key, const int *start, const int *sorted, int N, NnGrid gd,
                          float r2, int *deg, const int *rp, int *ci) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
     if (i >= N) return;
     const int k = key[i];
     const int sbase = k - k % gd.ncell;
@@ -801,10 +812,24 @@ int dmph_nn_gemm(const uint16_t *X, int ldx, const uint16_t *W, const float *bia
     if (ldy % 4 || (R && ldr % 4) || (relu & ~(DMPH_NN_RELU | DMPH_NN_F16_OUT))) return -1;   // 8-byte row words
     const dim3 grid((M + kRows - 1) / kRows, N / kCols);
     const size_t lds = (size_t)kCols * K * sizeof(bf16_t);   // <= 64 KB
+    hipStream_t s = (hipStream_t)stream;
+#define DMPH_GEMM(KS_, K_) hipLaunchKernelGGL((k_gemm<KS_, K_>), grid, dim3(256), lds, s, X, ldx, W, bias, R, ldr, Y, ldy, M, relu)
     if (kstep == 32) {
-        hipLaunchKernelGGL(k_gemm<32>, grid, dim3(256), lds, (hipStream_t)stream, X, ldx, W, bias, R, ldr, Y, ldy, M, K, relu);
-    } else
-        hipLaunchKernelGGL(k_gemm<16>, grid, dim3(256), lds, (hipStream_t)stream, X, ldx, W, bias, R, ldr, Y, ldy, M, K, relu);
+        switch (K) {
+            case 32: DMPH_GEMM(32, 32); break;
+            case 64: DMPH_GEMM(32, 64); break;
+            case 128: DMPH_GEMM(32, 128); break;
+            default: DMPH_GEMM(32, 256); break;
+        }
+    } else {
+        switch (K) {
+            case 32: DMPH_GEMM(16, 32); break;
+            case 64: DMPH_GEMM(16, 64); break;
+            case 128: DMPH_GEMM(16, 128); break;
+            default: DMPH_GEMM(16, 256); break;
+        }
+    }
+#undef DMPH_GEMM
     return launch_ok();
 }
 

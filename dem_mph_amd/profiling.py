@@ -7,7 +7,8 @@
   total duration per kernel);
 - HBM traffic per dispatch from two ``--pmc`` runs, FETCH_SIZE and WRITE_SIZE
   (KiB per dispatch), each scaled by what the same counter reads for a plain
-  device-to-device copy of known size (``bench.py --copy-probe MIB``): the copy
+  device-to-device copy of known size (``bench.py --copy-probe MIB``, or
+  ``python -m dem_mph_amd.surrogate --copy-probe MIB``): the copy
   kernel is the ``__amd_rocclr_copyBuffer`` dispatch with the largest count.
 """
 from __future__ import annotations
@@ -17,6 +18,24 @@ import collections
 import json
 import sqlite3
 from typing import Dict, List, Tuple
+
+
+def copy_probe(mib: int) -> None:
+    """One device-to-device copy of `mib` MiB through the HIP runtime: the copy
+    kernel's FETCH_SIZE / WRITE_SIZE under rocprof calibrate the counters
+    against a known byte count."""
+    import ctypes as C
+    from dem_mph_amd import share_hip_runtime
+    share_hip_runtime()
+    hip = C.CDLL("libamdhip64.so.7")   # by soname: the runtime already loaded, if any
+    n = mib << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(n)) == 0 and hip.hipMalloc(C.byref(b), C.c_size_t(n)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(n)) == 0
+    assert hip.hipMemcpy(b, a, C.c_size_t(n), 3) == 0   # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
 
 
 def kernel_stats(db: str) -> List[Tuple[str, int, float, float]]:

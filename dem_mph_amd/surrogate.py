@@ -709,9 +709,21 @@ def main(argv=None) -> int:
     ap.add_argument("--tol", type=float, default=0.008, help="debris-centroid RMS tolerance (1%% of the tank length)")
     ap.add_argument("--out", default=os.path.join(PKG, "surrogate_c3.npz"))
     ap.add_argument("--eval-only", action="store_true")
+    ap.add_argument("--throughput", type=int, default=0, metavar="SCENES",
+                    help="only time the saved model's rollout on SCENES ensemble scenes (seed 1), 3 steps")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="first one device-to-device copy of MIB MiB (counter calibration)")
     a = ap.parse_args(argv)
     log = lambda m: print("surrogate: " + m, file=sys.stderr, flush=True)
     params = S.params_c3()
+    if a.copy_probe:
+        from .profiling import copy_probe
+        copy_probe(a.copy_probe)
+    if a.throughput:
+        sc = S.ensemble_c3(a.throughput, seed=1)
+        r = advance_batch(load(a.out), sc, 3, params["Dt"])
+        print(json.dumps(r))
+        return 0
     rep = {}
     if not a.eval_only:
         sc = S.ensemble_c3(a.train_scenes, seed=a.seed)
