@@ -583,46 +583,67 @@ __global__ void k_g_scatter(const int *key, const int *start, int *fill, int *or
     order[start[k] + atomicAdd(&fill[k], 1)] = i;
 }
 
-// ascending node index inside each cell
-__global__ void k_g_rank(const int *key, const int *start, const int *order, int *sorted, int N) {
+// ascending node index inside each cell; ps[slot] = (x, y, z, node id bits)
+__global__ void k_g_rank(const int *key, const int *start, const int *order, const float *pos, float4 *ps, int N) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= N) return;
     const int i = order[p], k = key[i];
     int r = 0;
     for (int q = start[k]; q < start[k + 1]; q++) r += order[q] < i;
-    sorted[start[k] + r] = i;
+    ps[start[k] + r] = make_float4(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], __int_as_float(i));
 }
 
+// One wave per node: the 3 x 3 columns of neighbour cells are 9 contiguous runs
+// of the cell-sorted positions (the 3 cells along z are adjacent); lanes test
+// 64 candidates at a time and the hits of a step are stored contiguously
+// (ballot + prefix), so a row is written in whole runs.  FILL writes ci (and,
+// when eps is given, the edge features of edge_eps), else the degree.
 template <bool FILL>
-__global__ void k_g_rows(const float *pos, const int *key, const int *start, const int *sorted, int N, NnGrid gd,
-                         float r2, int *deg, const int *rp, int *ci) {
-    const int i = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(256) k_g_rows(const float *__restrict__ pos, const int *__restrict__ key,
+                                                const int *__restrict__ start, const float4 *__restrict__ ps, int N,
+                                                NnGrid gd, float r2, float inv_r, int *__restrict__ deg,
+                                                const int *__restrict__ rp, int *__restrict__ ci,
+                                                uint2 *__restrict__ eps) {
+    const int i = __builtin_amdgcn_readfirstlane(xcd_block(gridDim.x) * 4 + (threadIdx.x >> 6));
     if (i >= N) return;
+    const int lane = threadIdx.x & 63;
     const int k = key[i];
     const int sbase = k - k % gd.ncell;
     const int cell = k % gd.ncell;
     const int c[3] = {cell / (gd.cc[1] * gd.cc[2]), (cell / gd.cc[2]) % gd.cc[1], cell % gd.cc[2]};
     const float xi = pos[3 * i], yi = pos[3 * i + 1], zi = pos[3 * i + 2];
-    int cnt = 0, out = FILL ? rp[i] : 0;
-    for (int x = c[0] - 1; x <= c[0] + 1; x++) {
-        if (x < 0 || x >= gd.cc[0]) continue;
-        for (int y = c[1] - 1; y <= c[1] + 1; y++) {
-            if (y < 0 || y >= gd.cc[1]) continue;
-            for (int z = c[2] - 1; z <= c[2] + 1; z++) {
-                if (z < 0 || z >= gd.cc[2]) continue;
-                const int kk = sbase + (x * gd.cc[1] + y) * gd.cc[2] + z;
-                for (int q = start[kk]; q < start[kk + 1]; q++) {
-                    const int j = sorted[q];
-                    if (j == i) continue;
-                    const float dx = pos[3 * j] - xi, dy = pos[3 * j + 1] - yi, dz = pos[3 * j + 2] - zi;
-                    if (dx * dx + dy * dy + dz * dz > r2) continue;
-                    if (FILL) ci[out++] = j;
-                    cnt++;
+    const int z0 = max(c[2] - 1, 0), z1 = min(c[2] + 1, gd.cc[2] - 1);
+    int out = FILL ? rp[i] : 0;
+    for (int x = max(c[0] - 1, 0); x <= min(c[0] + 1, gd.cc[0] - 1); x++)
+        for (int y = max(c[1] - 1, 0); y <= min(c[1] + 1, gd.cc[1] - 1); y++) {
+            const int col = sbase + (x * gd.cc[1] + y) * gd.cc[2];
+            const int qe = start[col + z1 + 1];
+            for (int qq = start[col + z0]; qq < qe; qq += 64) {
+                const int q = qq + lane;
+                bool hit = false;
+                float dx = 0.f, dy = 0.f, dz = 0.f;
+                int j = -1;
+                if (q < qe) {
+                    const float4 p = ps[q];
+                    j = __float_as_int(p.w);
+                    dx = p.x - xi; dy = p.y - yi; dz = p.z - zi;
+                    hit = j != i && !(dx * dx + dy * dy + dz * dz > r2);
                 }
+                const uint64_t m = __ballot(hit);
+                if (FILL && hit) {
+                    const int o = out + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                                     __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+                    ci[o] = j;
+                    if (eps) {
+                        const float d = sqrtf(dx * dx + dy * dy + dz * dz);
+                        eps[o] = make_uint2(h2bits(h2_t{(_Float16)(dx * inv_r), (_Float16)(dy * inv_r)}),
+                                            h2bits(h2_t{(_Float16)(dz * inv_r), (_Float16)(d * inv_r)}));
+                    }
+                }
+                out += __popcll(m);
             }
         }
-    }
-    if (!FILL) deg[i] = cnt;
+    if (!FILL && lane == 0) deg[i] = out;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -928,7 +949,8 @@ int dmph_nn_bbox(const float *pos, int N, float *out, float *scratch, void *stre
 }
 
 int dmph_nn_graph(const float *pos, const int *scene, int N, const NnGrid *grid, float radius, int *key, int *counts,
-                  int *start, int *fill, int *order, int *sorted, int *deg, int *rp, int *ci, int cap, int *nnz_out,
+                  int *start, int *fill, int *order, int *sorted, int *deg, int *rp, int *ci, uint32_t *eps, int cap,
+                  int *nnz_out,
                   int *scratch, void *stream) {
     hipStream_t s = (hipStream_t)stream;
     const int nk = grid->ncell * (grid->nscene > 0 ? grid->nscene : 1);
@@ -938,17 +960,20 @@ int dmph_nn_graph(const float *pos, const int *scene, int N, const NnGrid *grid,
     hipLaunchKernelGGL(k_g_keys, dim3(b), dim3(256), 0, s, pos, scene, N, *grid, key, counts);
     scan_excl(counts, start, nk, scratch, s);
     hipLaunchKernelGGL(k_g_scatter, dim3(b), dim3(256), 0, s, key, start, fill, order, N);
-    hipLaunchKernelGGL(k_g_rank, dim3(b), dim3(256), 0, s, key, start, order, sorted, N);
+    float4 *ps = reinterpret_cast<float4 *>(sorted);
+    hipLaunchKernelGGL(k_g_rank, dim3(b), dim3(256), 0, s, key, start, order, pos, ps, N);
     const float r2 = radius * radius;
-    hipLaunchKernelGGL(k_g_rows<false>, dim3(b), dim3(256), 0, s, pos, key, start, sorted, N, *grid, r2, deg, nullptr,
-                       nullptr);
+    const int bw = (N + 3) / 4;   // a wave per node
+    hipLaunchKernelGGL(k_g_rows<false>, dim3(bw), dim3(256), 0, s, pos, key, start, ps, N, *grid, r2, 1.f / radius,
+                       deg, nullptr, nullptr, nullptr);
     scan_excl(deg, rp, N, scratch, s);
     int nnz = 0;
     if (hipMemcpyAsync(&nnz, rp + N, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess) return -5;
     if (hipStreamSynchronize(s) != hipSuccess) return -5;
     *nnz_out = nnz;
     if (nnz > cap) return -2;   // caller grows ci and calls again
-    hipLaunchKernelGGL(k_g_rows<true>, dim3(b), dim3(256), 0, s, pos, key, start, sorted, N, *grid, r2, deg, rp, ci);
+    hipLaunchKernelGGL(k_g_rows<true>, dim3(bw), dim3(256), 0, s, pos, key, start, ps, N, *grid, r2, 1.f / radius,
+                       deg, rp, ci, reinterpret_cast<uint2 *>(eps));
     return launch_ok();
 }
 

@@ -143,6 +143,7 @@ class Surrogate:
         self.stream = C.c_void_p(torch.cuda.current_stream(self.dev).cuda_stream)
         self._ws = None
         self._graph_ws = None
+        self.graph_eps = None   # edge features of the last graph(..., eps=True)
         self._bbox = None
         self._folded = None   # inference: per round [n1 left | n1 right @ w2] (rebuilt after a weight change)
 
@@ -190,8 +191,10 @@ class Surrogate:
         self._folded = None
 
     # ------------------------------------------------------------------ graph
-    def graph(self, pos, scene=None, nscene: int = 1) -> Tuple[object, object, int]:
-        """CSR radius graph (r <= 3 d) of the nodes at pos [N, 3] (FP32, device)."""
+    def graph(self, pos, scene=None, nscene: int = 1, eps: bool = False) -> Tuple[object, object, int]:
+        """CSR radius graph (r <= 3 d) of the nodes at pos [N, 3] (FP32, device);
+        with `eps` the edge features for inference are written alongside
+        (self.graph_eps, for forward(..., eps=...))."""
         torch = self.torch
         N = pos.shape[0]
         lib = nnlib()
@@ -215,24 +218,28 @@ class Surrogate:
         ws = self._graph_ws
         if ws is None or ws["N"] < N or ws["nk"] < nk:
             i32 = lambda n: torch.empty(max(n, 1), dtype=torch.int32, device=self.dev)
-            ws = {"N": N, "nk": nk, "key": i32(N), "order": i32(N), "sorted": i32(N), "deg": i32(N),
-                  "counts": i32(nk + 1), "start": i32(nk + 1), "fill": i32(nk + 1), "cap": 0, "ci": None,
+            ws = {"N": N, "nk": nk, "key": i32(N), "order": i32(N), "sorted": i32(4 * N), "deg": i32(N),
+                  "counts": i32(nk + 1), "start": i32(nk + 1), "fill": i32(nk + 1), "cap": 0, "ci": None, "eps": None,
                   "scan": i32(lib.dmph_nn_graph_scratch(N, nk) // 4 + 1)}
             self._graph_ws = ws
         rp = torch.empty(N + 1, dtype=torch.int32, device=self.dev)
         nnz = C.c_int(0)
         for _ in range(2):
             ci = ws["ci"]
+            if eps and ws["eps"] is None and ci is not None:
+                ws["eps"] = torch.empty(2 * ws["cap"], dtype=torch.int32, device=self.dev)
             rc = lib.dmph_nn_graph(_p(pos), _p(scene), N, C.byref(g), C.c_float(self.radius), _p(ws["key"]),
                                    _p(ws["counts"]), _p(ws["start"]), _p(ws["fill"]), _p(ws["order"]),
-                                   _p(ws["sorted"]), _p(ws["deg"]), _p(rp), _p(ci), ws["cap"], C.byref(nnz),
-                                   _p(ws["scan"]), stream)
+                                   _p(ws["sorted"]), _p(ws["deg"]), _p(rp), _p(ci), _p(ws["eps"] if eps else None),
+                                   ws["cap"], C.byref(nnz), _p(ws["scan"]), stream)
             if rc == -2:
                 ws["cap"] = int(nnz.value * 1.25) + 1024
                 ws["ci"] = torch.empty(ws["cap"], dtype=torch.int32, device=self.dev)
+                ws["eps"] = None
                 continue
             _chk(rc, "dmph_nn_graph")
             break
+        self.graph_eps = ws["eps"] if eps else None
         return rp, ws["ci"], int(nnz.value)
 
     # ------------------------------------------------------------------ forward / backward
@@ -319,7 +326,7 @@ class Surrogate:
             self._folded = out
         return self._folded
 
-    def forward(self, x0, pos, rp, ci, N: int, train: bool = True):
+    def forward(self, x0, pos, rp, ci, N: int, train: bool = True, eps=None):
         """Accelerations (scaled) y [N, 3] FP32; with `train` every round's
         activations are kept for backward, else two slots alternate and the
         aggregation's linear layer is folded into the node MLP (_fold)."""
@@ -332,11 +339,12 @@ class Surrogate:
         self._gemm(x0, FEAT, self.enc1, ws["h1"], W, N, True)
         self._gemm(ws["h1"], W, self.enc2, ws["hin"][0], 2 * W, N, False)
         folded = None if train else self._fold()
-        if not train:   # edge features once per graph, for the half-precision edge sums
+        if not train and eps is None:   # edge features once per graph, for the half-precision edge sums
             if ws.get("eps") is None or ws["eps"].numel() < 2 * ci.numel():
                 ws["eps"] = self.torch.empty(2 * ci.numel(), dtype=self.torch.int32, device=self.dev)
             _chk(lib.dmph_nn_edge_eps(_p(pos), _p(rp), _p(ci), _p(ws["eps"]), N, C.c_float(self.radius), s),
                  "edge_eps")
+            eps = ws["eps"]
         for r, L in enumerate(self.rounds):
             hin = ws["hin"][hslot(r)]
             ab, sr, u = ws["ab"][slot(r)], ws["s"][slot(r)], ws["u"][slot(r)]
@@ -348,7 +356,7 @@ class Surrogate:
                 self._gemm(sr, W, L["w2"], hin[:, W:], 2 * W, N, False)
                 self._gemm(hin, 2 * W, L["n1"], u, W, N, True)
             else:   # S straight into the right half of hin_r, then the folded node layer
-                _chk(lib.dmph_nn_edge_fwd_f16(_p(ab), _p(rp), _p(ci), _p(ws["eps"]), _p(L["c"].w), _p(L["c"].b),
+                _chk(lib.dmph_nn_edge_fwd_f16(_p(ab), _p(rp), _p(ci), _p(eps), _p(L["c"].w), _p(L["c"].b),
                                               _p(hin[:, W:]), 2 * W, N, s), "edge_fwd_f16")
                 self._gemm(hin, 2 * W, folded[r], u, W, N, True)
             self._gemm(u, W, L["n2"], ws["hin"][hslot(r + 1)], 2 * W, N, False, R=hin, ldr=2 * W)
@@ -583,9 +591,9 @@ def rollout(sur: Optional[Surrogate], traj: Trajectories, start: int, steps: int
     for k in range(1, steps + 1):
         t0 = _now()
         if baseline is None:
-            rp, ci, _ = sur.graph(pos, traj.scene, traj.nscene)
+            rp, ci, _ = sur.graph(pos, traj.scene, traj.nscene, eps=True)
             x0 = sur.features(vh, traj.cls, pos, train=False)
-            y = sur.forward(x0, pos, rp, ci, n, train=False)
+            y = sur.forward(x0, pos, rp, ci, n, train=False, eps=sur.graph_eps)
             ascale = sur.ascale
         else:
             a = np.zeros(3) if baseline == "still" else g
@@ -643,9 +651,9 @@ def advance_batch(sur: Surrogate, scenes, steps: int, dt: float) -> Dict[str, fl
     h = STRIDE * dt
 
     def one():
-        rp, ci, _ = sur.graph(pos, scene, len(scenes))
+        rp, ci, _ = sur.graph(pos, scene, len(scenes), eps=True)
         x0 = sur.features(vh, cls, pos, train=False)
-        y = sur.forward(x0, pos, rp, ci, n, train=False)
+        y = sur.forward(x0, pos, rp, ci, n, train=False, eps=sur.graph_eps)
         sur.rollout_step(pos, vh, y, cls, body, boff, h, body_sums)
 
     one()   # warm-up (workspace, graph capacity)

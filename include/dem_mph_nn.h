@@ -74,12 +74,14 @@ int dmph_nn_edge_bwd(const uint16_t *AB, const float *pos, const int *rp, const 
 
 /* radius graph (CSR rp[N + 1], ci[cap]) of the nodes at pos [N][3] (scene ids or NULL):
  * rows ascending by cell then node; *nnz_out = edges; returns -2 if cap < nnz (call again).
- * Work arrays: key, order, sorted, deg [N]; counts, start, fill [cells * scenes + 1];
- * scratch of dmph_nn_graph_scratch(N, cells * scenes) bytes (scan carries). */
+ * eps (NULL or [2 cap] words): also each edge's features as edge_eps writes them.
+ * Work arrays: key, order, deg [N], sorted [4N] (cell-sorted positions and ids);
+ * counts, start, fill [cells * scenes + 1]; scratch of
+ * dmph_nn_graph_scratch(N, cells * scenes) bytes (scan carries). */
 size_t dmph_nn_graph_scratch(int N, int ncells_total);
 int dmph_nn_graph(const float *pos, const int *scene, int N, const NnGrid *grid, float radius, int *key, int *counts,
-                  int *start, int *fill, int *order, int *sorted, int *deg, int *rp, int *ci, int cap, int *nnz_out,
-                  int *scratch, void *stream);
+                  int *start, int *fill, int *order, int *sorted, int *deg, int *rp, int *ci, uint32_t *eps, int cap,
+                  int *nnz_out, int *scratch, void *stream);
 
 /* bounding box of pos [N][3] into out[6] (min x y z, max x y z; device memory);
  * scratch of dmph_nn_bbox_scratch(N) bytes */

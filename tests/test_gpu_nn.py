@@ -94,7 +94,15 @@ def test_radius_graph_matches_brute_force(S):
     rng = np.random.default_rng(5)
     pos = rng.uniform(0.0, 0.3, (3000, 3)).astype(np.float32)
     sur = S.Surrogate(d=0.01, tank_lo=[0, 0, 0], tank_hi=[0.3, 0.3, 0.3], vscale=1.0, ascale=1.0)
-    rp, ci, nnz = sur.graph(torch.tensor(pos, device="cuda:0"))
+    posd = torch.tensor(pos, device="cuda:0")
+    rp, ci, nnz = sur.graph(posd, eps=True)
+    # the edge features written with the graph equal edge_eps's
+    e1 = sur.graph_eps[:2 * nnz].cpu().numpy().view(np.float16).astype(np.float64)
+    e2 = torch.empty(2 * ci.numel(), dtype=torch.int32, device="cuda:0")
+    s = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    assert S.nnlib().dmph_nn_edge_eps(_p(posd), _p(rp), _p(ci), _p(e2), len(pos), C.c_float(sur.radius), s) == 0
+    e2 = e2[:2 * nnz].cpu().numpy().view(np.float16).astype(np.float64)
+    assert np.abs(e1 - e2).max() <= 1e-3 and np.abs(e1).max() <= 1.0 + 1e-3
     rp, ci = rp.cpu().numpy(), ci.cpu().numpy()[:nnz]
     brp, bci = NO.radius_graph(pos.astype(np.float64), sur.radius)
     # the GPU tests r^2 in FP32: pairs within a rounding of the radius may differ
