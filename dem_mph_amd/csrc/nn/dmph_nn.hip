@@ -383,20 +383,25 @@ __global__ void __launch_bounds__(256) k_edge_fwd_h(const h2_t *__restrict__ AB,
         c = __builtin_elementwise_fma(wd, __builtin_shufflevector(ezd, ezd, 1, 1), c);
         return acc + __builtin_elementwise_maximum(c, zero);   // NaN-propagating: an overflow stays visible
     };
-    for (; q + kEdgeBatch <= q1; q += kEdgeBatch) {
-        h2_t bj[kEdgeBatch];
+    // batches of B edges: all rows requested, then the chains (which interleave)
+    auto batch = [&](auto nb) {
+        constexpr int B = decltype(nb)::value;
+        h2_t bj[B];
 #pragma unroll
-        for (int t = 0; t < kEdgeBatch; t++) bj[t] = b_of(ci[q + t]);
+        for (int t = 0; t < B; t++) bj[t] = b_of(ci[q + t]);
         h2_t acc = zero;
 #pragma unroll
-        for (int t = 0; t < kEdgeBatch; t++) acc = edge(bj[t], eps[q + t], acc);
+        for (int t = 0; t < B; t++) acc = edge(bj[t], eps[q + t], acc);
         s0 += (float)acc.x;
         s1 += (float)acc.y;
-    }
-    h2_t acc = zero;
-    for (; q < q1; q++) acc = edge(b_of(ci[q]), eps[q], acc);
-    s0 += (float)acc.x;
-    s1 += (float)acc.y;
+        q += B;
+    };
+    while (q + kEdgeBatch <= q1) batch(std::integral_constant<int, kEdgeBatch>{});
+    // the rest (< 16) in batches of 8, 4, 2, 1: at most four round trips
+    if (q + 8 <= q1) batch(std::integral_constant<int, 8>{});
+    if (q + 4 <= q1) batch(std::integral_constant<int, 4>{});
+    if (q + 2 <= q1) batch(std::integral_constant<int, 2>{});
+    if (q < q1) batch(std::integral_constant<int, 1>{});
     *reinterpret_cast<uint32_t *>(S + (size_t)i * lds + f) = (uint32_t)f2bf(s0) | ((uint32_t)f2bf(s1) << 16);
 }
 
