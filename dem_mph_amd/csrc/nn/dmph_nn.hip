@@ -61,7 +61,6 @@ __device__ __forceinline__ int xcd_block(int G) {
     return x * q + min(x, r) + (w >> 3);
 }
 
-constexpr int kRows = 128;
 constexpr int kCols = 128;
 
 // LDS position of W[r][k]: 16-byte granule k / 8 of row r is stored at granule
@@ -72,22 +71,36 @@ __device__ __forceinline__ int wl_at(int r, int k, int K) {
     return r * K + (((k >> 3) ^ (r & m)) << 3) + (k & 7);
 }
 
-template <int KS, int K>
-__global__ void __launch_bounds__(256) k_gemm(const bf16_t *__restrict__ X, int ldx, const bf16_t *__restrict__ W,
-                                              const float *__restrict__ bias, const bf16_t *__restrict__ R, int ldr,
-                                              bf16_t *__restrict__ Y, int ldy, int M, int flags) {
-    extern __shared__ bf16_t wl[];   // [128][K], 16-byte granules XOR-swizzled by row (wl_at)
-    constexpr int NK = K / KS;
+// Workgroup shape, measured on the 10M-row surrogate layers (ms per layer,
+// K x N = 128 x 256 / 256 x 128 / 128 x 128 + residual): 2 row blocks per wave,
+// 4 waves, 1 column block: 3.08 / 3.05 / 2.14; 1, 8, 1: 2.91 / 2.16 / 2.00;
+// 1, 8, 2: 2.18 / 2.18 / 2.02 (X read once for both halves of N = 256).
+#ifndef DMPH_GEMM_RB
+#define DMPH_GEMM_RB 1   // 16-row blocks per wave
+#endif
+#ifndef DMPH_GEMM_NW
+#define DMPH_GEMM_NW 8   // waves per workgroup
+#endif
+#ifndef DMPH_GEMM_NC
+#define DMPH_GEMM_NC 2   // 128-column blocks per workgroup when N = 256 (and K <= 128)
+#endif
+
+template <int KS, int K, int RB, int NW, int NC>
+__global__ void __launch_bounds__(NW * 64) k_gemm(const bf16_t *__restrict__ X, int ldx, const bf16_t *__restrict__ W,
+                                                  const float *__restrict__ bias, const bf16_t *__restrict__ R, int ldr,
+                                                  bf16_t *__restrict__ Y, int ldy, int M, int flags) {
+    extern __shared__ bf16_t wl[];   // [NC * 128][K], 16-byte granules XOR-swizzled by row (wl_at)
+    constexpr int NK = K / KS, CT = 8 * NC;   // k steps, 16-column tiles
     typedef typename std::conditional<KS == 32, uint4, uint2>::type frag_t;
-    const int n0 = blockIdx.y * kCols;
+    const int n0 = blockIdx.y * kCols * NC;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int row0 = blockIdx.x * kRows + wave * 32;
+    const int row0 = (blockIdx.x * NW + wave) * RB * 16;
     const int lr = lane & 15, lk = (lane >> 4) * (KS / 4);
-    // every X fragment of the wave's two 16-row blocks, requested before the W
+    // every X fragment of the wave's 16-row blocks, requested before the W
     // block is staged so the two latencies overlap (rows past M read as zero)
-    frag_t xf[NK][2];
+    frag_t xf[NK][RB];
 #pragma unroll
-    for (int a = 0; a < 2; a++) {
+    for (int a = 0; a < RB; a++) {
         const int r = row0 + a * 16 + lr;
 #pragma unroll
         for (int kk = 0; kk < NK; kk++) {
@@ -95,24 +108,24 @@ __global__ void __launch_bounds__(256) k_gemm(const bf16_t *__restrict__ X, int 
             if (r < M) xf[kk][a] = *reinterpret_cast<const frag_t *>(X + (size_t)r * ldx + kk * KS + lk);
         }
     }
-    for (int t = threadIdx.x; t < kCols * K / 8; t += blockDim.x) {   // 16-byte copies
+    for (int t = threadIdx.x; t < CT * 16 * K / 8; t += blockDim.x) {   // 16-byte copies
         const int r = t / (K / 8), c = (t % (K / 8)) * 8;
         *reinterpret_cast<uint4 *>(wl + wl_at(r, c, K)) = *reinterpret_cast<const uint4 *>(W + (size_t)(n0 + r) * K + c);
     }
     __syncthreads();
-    f4_t acc[2][8];
+    f4_t acc[RB][CT];
 #pragma unroll
-    for (int a = 0; a < 2; a++)
+    for (int a = 0; a < RB; a++)
 #pragma unroll
-        for (int c = 0; c < 8; c++) acc[a][c] = f4_t{0.f, 0.f, 0.f, 0.f};
+        for (int c = 0; c < CT; c++) acc[a][c] = f4_t{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int kk = 0; kk < NK; kk++) {
         const int k = kk * KS;
 #pragma unroll
-        for (int c = 0; c < 8; c++) {
+        for (int c = 0; c < CT; c++) {
             const frag_t v = *reinterpret_cast<const frag_t *>(wl + wl_at(c * 16 + lr, k + lk, K));
 #pragma unroll
-            for (int a = 0; a < 2; a++) {
+            for (int a = 0; a < RB; a++) {
                 if constexpr (KS == 16)
                     acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(__builtin_bit_cast(s4_t, v),
                                                                           __builtin_bit_cast(s4_t, xf[kk][a]), acc[a][c], 0, 0, 0);
@@ -125,21 +138,21 @@ __global__ void __launch_bounds__(256) k_gemm(const bf16_t *__restrict__ X, int 
     // epilogue: bias, residual, ReLU; acc[a][c][e] is row row0 + 16a + lr, column
     // n0 + 16c + 4 (lane / 16) + e
     const bool relu = flags & DMPH_NN_RELU, half = flags & DMPH_NN_F16_OUT;
-    uint2 rr[2][8];   // residual words, all requested before the first is used
+    uint2 rr[RB][CT];   // residual words, all requested before the first is used
 #pragma unroll
-    for (int a = 0; a < 2; a++)
+    for (int a = 0; a < RB; a++)
 #pragma unroll
-        for (int c = 0; c < 8; c++) {
+        for (int c = 0; c < CT; c++) {
             const int r = row0 + a * 16 + lr;
             rr[a][c] = make_uint2(0, 0);
             if (R && r < M) rr[a][c] = *reinterpret_cast<const uint2 *>(R + (size_t)r * ldr + n0 + c * 16 + (lane >> 4) * 4);
         }
 #pragma unroll
-    for (int c = 0; c < 8; c++) {
+    for (int c = 0; c < CT; c++) {
         const int col = n0 + c * 16 + (lane >> 4) * 4;
         const float4 b = bias ? *reinterpret_cast<const float4 *>(bias + col) : make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
-        for (int a = 0; a < 2; a++) {
+        for (int a = 0; a < RB; a++) {
             const int r = row0 + a * 16 + lr;
             if (r >= M) continue;
             float v[4] = {acc[a][c][0] + b.x, acc[a][c][1] + b.y, acc[a][c][2] + b.z, acc[a][c][3] + b.w};
@@ -836,10 +849,21 @@ int dmph_nn_gemm(const uint16_t *X, int ldx, const uint16_t *W, const float *bia
     // K / 8 must be a power of two (the LDS swizzle stays inside a row)
     if ((K != 32 && K != 64 && K != 128 && K != 256) || N % 128 || (kstep != 16 && kstep != 32)) return -1;
     if (ldy % 4 || (R && ldr % 4) || (relu & ~(DMPH_NN_RELU | DMPH_NN_F16_OUT))) return -1;   // 8-byte row words
-    const dim3 grid((M + kRows - 1) / kRows, N / kCols);
-    const size_t lds = (size_t)kCols * K * sizeof(bf16_t);   // <= 64 KB
     hipStream_t s = (hipStream_t)stream;
-#define DMPH_GEMM(KS_, K_) hipLaunchKernelGGL((k_gemm<KS_, K_>), grid, dim3(256), lds, s, X, ldx, W, bias, R, ldr, Y, ldy, M, relu)
+    constexpr int RB = DMPH_GEMM_RB, NW = DMPH_GEMM_NW, NCW = DMPH_GEMM_NC;
+    constexpr int rows = RB * NW * 16;
+    const int nc = (N == 256 && K <= 128) ? NCW : 1;   // W block in LDS <= 64 KB
+    const dim3 grid((M + rows - 1) / rows, N / (kCols * nc));
+    const size_t lds = (size_t)kCols * nc * K * sizeof(bf16_t);   // <= 64 KB
+#define DMPH_GEMM(KS_, K_)                                                                                              \
+    do {                                                                                                              \
+        if (nc == 2)                                                                                                  \
+            hipLaunchKernelGGL((k_gemm<KS_, K_, RB, NW, 2>), grid, dim3(NW * 64), lds, s, X, ldx, W, bias, R, ldr, Y,   \
+                               ldy, M, relu);                                                                         \
+        else                                                                                                          \
+            hipLaunchKernelGGL((k_gemm<KS_, K_, RB, NW, 1>), grid, dim3(NW * 64), lds, s, X, ldx, W, bias, R, ldr, Y,   \
+                               ldy, M, relu);                                                                         \
+    } while (0)
     if (kstep == 32) {
         switch (K) {
             case 32: DMPH_GEMM(32, 32); break;
