@@ -85,10 +85,15 @@ __device__ __forceinline__ int wl_at(int r, int k, int K) {
 #define DMPH_GEMM_NC 2   // 128-column blocks per workgroup when N = 256 (and K <= 128)
 #endif
 
-template <int KS, int K, int RB, int NW, int NC>
+// HEAD: the layer is the decoder's hidden layer (N = 128, ReLU); instead of
+// storing it, each row's bf16-rounded values go through the 3-output head,
+// Y3[r] = Wd g_r + bd (FP32), summed across the four lanes that hold the row.
+template <int KS, int K, int RB, int NW, int NC, bool HEAD = false>
 __global__ void __launch_bounds__(NW * 64) k_gemm(const bf16_t *__restrict__ X, int ldx, const bf16_t *__restrict__ W,
                                                   const float *__restrict__ bias, const bf16_t *__restrict__ R, int ldr,
-                                                  bf16_t *__restrict__ Y, int ldy, int M, int flags) {
+                                                  bf16_t *__restrict__ Y, int ldy, int M, int flags,
+                                                  const float *__restrict__ Wd = nullptr,
+                                                  const float *__restrict__ bd = nullptr, float *__restrict__ Y3 = nullptr) {
     extern __shared__ bf16_t wl[];   // [NC * 128][K], 16-byte granules XOR-swizzled by row (wl_at)
     constexpr int NK = K / KS, CT = 8 * NC;   // k steps, 16-column tiles
     typedef typename std::conditional<KS == 32, uint4, uint2>::type frag_t;
@@ -138,6 +143,35 @@ __global__ void __launch_bounds__(NW * 64) k_gemm(const bf16_t *__restrict__ X, 
     // epilogue: bias, residual, ReLU; acc[a][c][e] is row row0 + 16a + lr, column
     // n0 + 16c + 4 (lane / 16) + e
     const bool relu = flags & DMPH_NN_RELU, half = flags & DMPH_NN_F16_OUT;
+    if constexpr (HEAD) {
+#pragma unroll
+        for (int a = 0; a < RB; a++) {
+            const int r = row0 + a * 16 + lr;
+            float part[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+            for (int c = 0; c < CT; c++) {
+                const int col = c * 16 + (lane >> 4) * 4;
+                const float4 b = *reinterpret_cast<const float4 *>(bias + col);
+                const float bb[4] = {b.x, b.y, b.z, b.w};
+#pragma unroll
+                for (int o = 0; o < 3; o++) {
+                    const float4 w = *reinterpret_cast<const float4 *>(Wd + o * 128 + col);
+                    const float ww[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+                    for (int e = 0; e < 4; e++) part[o] += bf2f(f2bf(fmaxf(acc[a][c][e] + bb[e], 0.f))) * ww[e];
+                }
+            }
+#pragma unroll
+            for (int o = 0; o < 3; o++) {
+                part[o] += __shfl_xor(part[o], 16);
+                part[o] += __shfl_xor(part[o], 32);
+            }
+            if (lane < 16 && r < M)
+#pragma unroll
+                for (int o = 0; o < 3; o++) Y3[3 * (size_t)r + o] = part[o] + bd[o];
+        }
+        return;
+    }
     uint2 rr[RB][CT];   // residual words, all requested before the first is used
 #pragma unroll
     for (int a = 0; a < RB; a++)
@@ -880,6 +914,23 @@ int dmph_nn_gemm(const uint16_t *X, int ldx, const uint16_t *W, const float *bia
         }
     }
 #undef DMPH_GEMM
+    return launch_ok();
+}
+
+int dmph_nn_gemm_head(const uint16_t *X, int ldx, const uint16_t *W, const float *bias, const float *Wd,
+                      const float *bd, float *Y3, int M, int kstep, void *stream) {
+    if (M <= 0) return 0;
+    if (kstep != 16 && kstep != 32) return -1;
+    constexpr int RB = DMPH_GEMM_RB, NW = DMPH_GEMM_NW, rows = RB * NW * 16;
+    const dim3 grid((M + rows - 1) / rows, 1);
+    const size_t lds = (size_t)kCols * 128 * sizeof(bf16_t);
+    hipStream_t s = (hipStream_t)stream;
+    if (kstep == 32)
+        hipLaunchKernelGGL((k_gemm<32, 128, RB, NW, 1, true>), grid, dim3(NW * 64), lds, s, X, ldx, W, bias, nullptr, 0,
+                           nullptr, 0, M, DMPH_NN_RELU, Wd, bd, Y3);
+    else
+        hipLaunchKernelGGL((k_gemm<16, 128, RB, NW, 1, true>), grid, dim3(NW * 64), lds, s, X, ldx, W, bias, nullptr, 0,
+                           nullptr, 0, M, DMPH_NN_RELU, Wd, bd, Y3);
     return launch_ok();
 }
 

@@ -360,8 +360,12 @@ class Surrogate:
                                               _p(hin[:, W:]), 2 * W, N, s), "edge_fwd_f16")
                 self._gemm(hin, 2 * W, folded[r], u, W, N, True)
             self._gemm(u, W, L["n2"], ws["hin"][hslot(r + 1)], 2 * W, N, False, R=hin, ldr=2 * W)
-        self._gemm(ws["hin"][hslot(ROUNDS)], 2 * W, self.dec1, ws["g"], W, N, True)
-        _chk(lib.dmph_nn_head(_p(ws["g"]), _p(self.dec2.w), _p(self.dec2.b), _p(ws["y"]), N, s), "head")
+        if train:
+            self._gemm(ws["hin"][hslot(ROUNDS)], 2 * W, self.dec1, ws["g"], W, N, True)
+            _chk(lib.dmph_nn_head(_p(ws["g"]), _p(self.dec2.w), _p(self.dec2.b), _p(ws["y"]), N, s), "head")
+        else:   # the hidden layer never leaves the registers
+            _chk(lib.dmph_nn_gemm_head(_p(ws["hin"][hslot(ROUNDS)]), 2 * W, _p(self.dec1.wb), _p(self.dec1.b),
+                                       _p(self.dec2.w), _p(self.dec2.b), _p(ws["y"]), N, self.kstep, s), "gemm_head")
         return ws["y"][:N]
 
     def loss_and_backward(self, x0, pos, rp, ci, N: int, target, mask, count: int, clip: float = 1e30) -> float:

@@ -44,6 +44,12 @@ typedef struct NnFeat {
 int dmph_nn_gemm(const uint16_t *X, int ldx, const uint16_t *W, const float *bias, const uint16_t *R, int ldr,
                  uint16_t *Y, int ldy, int M, int K, int N, int flags, int kstep, void *stream);
 
+/* decoder tail for inference: g = relu(X[M][128] W^T + bias) (W [128][128]) rounded
+ * to bf16 as dmph_nn_gemm stores it, then Y3 [M][3] = g Wd^T + bd (Wd [3][128]) as
+ * dmph_nn_head computes it; g is never stored */
+int dmph_nn_gemm_head(const uint16_t *X, int ldx, const uint16_t *W, const float *bias, const float *Wd,
+                      const float *bd, float *Y3, int M, int kstep, void *stream);
+
 /* dW[N][K] (+)= dY^T X, db[N] (+)= sum_m dY (db may be NULL); K in {32, 128, 256},
  * N % 128 == 0; `groups` row chunks summed in a fixed order (deterministic);
  * scratch of dmph_nn_dw_scratch bytes */

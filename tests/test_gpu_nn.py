@@ -72,6 +72,33 @@ def test_gemm_matches_numpy(S, kstep, K, N, relu, res, half):
     assert np.abs(got - ref).mean() <= tol / 5 * np.abs(ref).mean() + 1e-6
 
 
+@pytest.mark.parametrize("kstep", [16, 32])
+def test_gemm_head_matches_gemm_then_head(S, kstep):
+    """Inference decoder tail (hidden layer + 3-output head in one kernel)
+    against the two kernels the training path runs, on the same bf16 inputs;
+    M = 301 leaves a partial tile; X with a row stride of 256."""
+    rng = np.random.default_rng(kstep)
+    M = 301
+    X = rng.standard_normal((M, 256)).astype(np.float32)
+    Wt = (rng.standard_normal((128, 128)) / np.sqrt(128)).astype(np.float32)
+    b = rng.standard_normal(128).astype(np.float32)
+    Wd = rng.standard_normal((3, 128)).astype(np.float32)
+    bd = rng.standard_normal(3).astype(np.float32)
+    Xd, Wdv = _dev_bf16(X), _dev_bf16(Wt)
+    bdev, Wdd, bdd = (torch.tensor(a, device="cuda:0") for a in (b, Wd, bd))
+    lib = S.nnlib()
+    s = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    G = torch.zeros((M, 128), dtype=torch.int16, device="cuda:0")
+    y1 = torch.zeros((M, 3), dtype=torch.float32, device="cuda:0")
+    y2 = torch.full((M, 3), float("nan"), dtype=torch.float32, device="cuda:0")
+    assert lib.dmph_nn_gemm(_p(Xd), 256, _p(Wdv), _p(bdev), None, 0, _p(G), 128, M, 128, 128, S.GEMM_RELU, kstep, s) == 0
+    assert lib.dmph_nn_head(_p(G), _p(Wdd), _p(bdd), _p(y1), M, s) == 0
+    assert lib.dmph_nn_gemm_head(_p(Xd), 256, _p(Wdv), _p(bdev), _p(Wdd), _p(bdd), _p(y2), M, kstep, s) == 0
+    a, c = y1.cpu().numpy(), y2.cpu().numpy()
+    assert np.isfinite(c).all()
+    np.testing.assert_allclose(c, a, rtol=1e-5, atol=1e-5 * np.abs(a).max())
+
+
 @pytest.mark.parametrize("K,N", [(32, 128), (128, 128), (256, 128), (128, 256)])
 def test_gemm_dw_matches_numpy(S, K, N):
     rng = np.random.default_rng(K * N)
