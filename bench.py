@@ -239,6 +239,55 @@ def ensemble_bench(D, nscenes: int, steps: int, warmup: int, seed: int, sample: 
 SURROGATE_WEIGHTS = os.path.join(ROOT, "dem_mph_amd", "surrogate_c3.npz")
 
 
+def channel_bench(D, steps: int, warmup: int, seed: int, surrogate: bool):
+    """The large single scene (C5 channel, 12.9M particles): the mixed solver
+    timed over `steps` replays of the captured step after autotuning, and the
+    learned surrogate's rollout throughput on the same particles (the network is
+    trained on C3 scenes; its accuracy on this geometry is not evaluated, so the
+    plain solver is the fallback that would take over)."""
+    from dem_mph_amd import scenes as S
+    t0 = time.perf_counter()
+    sc = S.channel_c5(seed=seed)
+    t_gen = time.perf_counter() - t0
+    params = S.params_c5()
+    dt = params["Dt"]
+    n = sc.n
+    cnt = sc.counts()
+    out = {"workload": "channel_c5: 1.6x0.8x0.8 m channel, d = 0.002 m, %d fluid (released column), %d debris "
+                       "in %d rigid cuboids of side 4-12, %d wall, dt = 1e-5 s" % (cnt["fluid"], cnt["solid"],
+                                                                                  cnt["bodies"], cnt["wall"]),
+           "particles": n, "steps": steps, "generation_s": t_gen}
+    t0 = time.perf_counter()
+    s = D.Solver(D.Params.from_dict(params), sc, precision=D.PREC_FP32, rigid=True, compensated=True, graph=True)
+    out["create_s"] = time.perf_counter() - t0
+    s.step(warmup)
+    s.sync()
+    out["autotuned_lanes"] = list(s.autotune(3))
+    s.step(2)
+    s.sync()
+    reb0 = s.info().rebuilds
+    t0 = time.perf_counter()
+    s.step(steps)
+    s.sync()
+    el = time.perf_counter() - t0
+    out["rebuilds_in_timed_region"] = int(s.info().rebuilds - reb0)
+    s.close()
+    out["mixed"] = {"particle_steps_per_s": n * steps / el, "ms_per_step": el / steps * 1e3,
+                    "sim_s_per_wall_s": dt * steps / el}
+    print("bench: channel mixed %.2f ms/step" % (el / steps * 1e3), file=sys.stderr, flush=True)
+    if surrogate and os.path.exists(SURROGATE_WEIGHTS):
+        from dem_mph_amd import surrogate as SG
+        # the graph radius follows this scene's spacing (3 d), as in training
+        r = SG.advance_batch(SG.load(SURROGATE_WEIGHTS).at_spacing(sc.d), [sc], 2, dt)
+        t = r["seconds_per_step"]
+        out["surrogate"] = {"ms_per_step": t * 1e3, "stride_dt": SG.STRIDE, "finite": r["finite"],
+                            "sim_s_per_wall_s": SG.STRIDE * dt / t, "particle_steps_per_s": n * SG.STRIDE / t,
+                            "vs_mixed": (SG.STRIDE * dt / t) / (dt * steps / el),
+                            "accuracy": "not evaluated on this geometry (trained on C3 scenes)"}
+        print("bench: channel surrogate %.1f ms/step" % (t * 1e3), file=sys.stderr, flush=True)
+    return out
+
+
 def surrogate_bench(D, scenes, seq_ms_per_dt: float, batched_ms_per_dt: float, chunk: int, steps: int,
                     eval_scenes: int, eval_steps: int, seed: int):
     """The learned surrogate on the ensemble: throughput of surrogate + fallback
@@ -290,6 +339,8 @@ def main(argv=None) -> int:
     ap.add_argument("--no-surrogate", action="store_true", help="skip the learned-surrogate measurement")
     ap.add_argument("--surrogate-chunk", type=int, default=128, help="scenes per surrogate batch")
     ap.add_argument("--surrogate-eval-scenes", type=int, default=16)
+    ap.add_argument("--no-channel", action="store_true", help="skip the large-channel (C5) measurement")
+    ap.add_argument("--channel-steps", type=int, default=100)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="also run one device-to-device copy of MIB MiB (counter calibration)")
     a = ap.parse_args(argv)
@@ -375,6 +426,9 @@ def main(argv=None) -> int:
                 D, scenes, ens["sequential_plain"]["ms_per_step"], ens["batched_mixed"]["ms_per_step"],
                 a.surrogate_chunk, 3, a.surrogate_eval_scenes, 100, a.seed)
         del scenes
+    if not a.no_channel:
+        print("bench: large channel", file=sys.stderr, flush=True)
+        out["channel"] = channel_bench(D, a.channel_steps, 5, a.seed, not a.no_surrogate)
     print(json.dumps(out))
     return 0
 

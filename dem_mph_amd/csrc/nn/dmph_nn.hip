@@ -635,6 +635,14 @@ __global__ void k_g_scatter(const int *key, const int *start, int *fill, int *or
     order[start[k] + atomicAdd(&fill[k], 1)] = i;
 }
 
+// 64-bit sum of n non-negative ints into *out (zeroed by the caller)
+__global__ void __launch_bounds__(256) k_sum64(const int *__restrict__ v, int n, unsigned long long *out) {
+    unsigned long long t = 0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) t += (unsigned)v[i];
+    for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o);
+    if ((threadIdx.x & 63) == 0) atomicAdd(out, t);
+}
+
 // ascending node index inside each cell; ps[slot] = (x, y, z, node id bits)
 __global__ void k_g_rank(const int *key, const int *start, const int *order, const float *pos, float4 *ps, int N) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1013,9 +1021,10 @@ static void scan_excl(const int *in, int *out, int n, int *scratch, hipStream_t 
     hipLaunchKernelGGL(k_scan_tiles, dim3(nt), dim3(256), 0, s, in, out, n, scratch + nt + 1);
 }
 
+// scratch layout: [0, 2) ints the 64-bit edge total, then the scan carries
 size_t dmph_nn_graph_scratch(int N, int ncells_total) {
     const int n = N > ncells_total ? N : ncells_total;
-    return (size_t)(2 * ((n + kTile - 1) / kTile) + 2) * sizeof(int);
+    return (size_t)(2 * ((n + kTile - 1) / kTile) + 4) * sizeof(int);
 }
 
 size_t dmph_nn_bbox_scratch(int N) { (void)N; return 256 * 6 * sizeof(float); }
@@ -1038,7 +1047,10 @@ int dmph_nn_graph(const float *pos, const int *scene, int N, const NnGrid *grid,
     if (hipMemsetAsync(fill, 0, (size_t)(nk + 1) * sizeof(int), s) != hipSuccess) return -5;
     const int b = (N + 255) / 256;
     hipLaunchKernelGGL(k_g_keys, dim3(b), dim3(256), 0, s, pos, scene, N, *grid, key, counts);
-    scan_excl(counts, start, nk, scratch, s);
+    unsigned long long *total = reinterpret_cast<unsigned long long *>(scratch);
+    int *carries = scratch + 2;
+    if (hipMemsetAsync(total, 0, sizeof(unsigned long long), s) != hipSuccess) return -5;
+    scan_excl(counts, start, nk, carries, s);
     hipLaunchKernelGGL(k_g_scatter, dim3(b), dim3(256), 0, s, key, start, fill, order, N);
     float4 *ps = reinterpret_cast<float4 *>(sorted);
     hipLaunchKernelGGL(k_g_rank, dim3(b), dim3(256), 0, s, key, start, order, pos, ps, N);
@@ -1046,10 +1058,17 @@ int dmph_nn_graph(const float *pos, const int *scene, int N, const NnGrid *grid,
     const int bw = (N + 3) / 4;   // a wave per node
     hipLaunchKernelGGL(k_g_rows<false>, dim3(bw), dim3(256), 0, s, pos, key, start, ps, N, *grid, r2, 1.f / radius,
                        deg, nullptr, nullptr, nullptr);
-    scan_excl(deg, rp, N, scratch, s);
+    scan_excl(deg, rp, N, carries, s);
+    hipLaunchKernelGGL(k_sum64, dim3(b < 1024 ? b : 1024), dim3(256), 0, s, deg, N, total);
     int nnz = 0;
+    unsigned long long all = 0;
     if (hipMemcpyAsync(&nnz, rp + N, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess) return -5;
+    if (hipMemcpyAsync(&all, total, sizeof(all), hipMemcpyDeviceToHost, s) != hipSuccess) return -5;
     if (hipStreamSynchronize(s) != hipSuccess) return -5;
+    if (all > 0x7fffffffull) {   // int32 row offsets would wrap: refuse before anything is written
+        *nnz_out = -1;
+        return -3;
+    }
     *nnz_out = nnz;
     if (nnz > cap) return -2;   // caller grows ci and calls again
     hipLaunchKernelGGL(k_g_rows<true>, dim3(bw), dim3(256), 0, s, pos, key, start, ps, N, *grid, r2, 1.f / radius,

@@ -211,7 +211,10 @@ def _pad_domain(lo: np.ndarray, hi: np.ndarray, d: float, max_radius_ratio: floa
 
 
 def tank_walls(interior: Sequence[float], d: float, layers: int, closed_top: bool, dim: int) -> np.ndarray:
-    """Wall lattice points in ``layers`` shells around the box [0, interior)."""
+    """Wall lattice points in ``layers`` shells around the box [0, interior):
+    the lattice of the padded box without its interior points, in lattice
+    order (x slowest), built one x plane at a time so that large tanks never
+    hold the full box lattice."""
     L = np.asarray(interior, float)
     lo = -layers * d * np.ones(3); hi = L + layers * d
     if not closed_top:
@@ -219,11 +222,18 @@ def tank_walls(interior: Sequence[float], d: float, layers: int, closed_top: boo
         hi[up] = L[up]
     if dim == 2:
         lo[2] = 0.0; hi[2] = d
-    pts = lattice(lo, hi, d, dim)
-    inside = np.ones(len(pts), bool)
-    for a in range(dim):
-        inside &= (pts[:, a] > 0.0) & (pts[:, a] < L[a])
-    return pts[~inside]
+    full = lattice(np.array([lo[0], lo[1], lo[2]]), np.array([lo[0] + d, hi[1], hi[2]]), d, dim)   # one x plane
+    nx = int(round((hi[0] - lo[0]) / d))
+    inner_yz = np.ones(len(full), bool)
+    for a in range(1, dim):
+        inner_yz &= (full[:, a] > 0.0) & (full[:, a] < L[a])
+    out = []
+    for k in range(nx):
+        x = lo[0] + (k + 0.5) * d
+        pts = full.copy()
+        pts[:, 0] = x
+        out.append(pts if not (0.0 < x < L[0]) else pts[~inner_yz])
+    return np.concatenate(out)
 
 
 def dam_break(d: float, interior: Sequence[float], fluid: Sequence[float], layers: int = 3,
@@ -377,6 +387,43 @@ def ensemble_c3(nscenes: int = 512, seed: int = 0, d: float = 0.01,
     return out
 
 
+def channel_c5(seed: int = 0, rigid: bool = True, scale: float = 1.0, nbodies: int = 2048) -> Scene:
+    """Large single scene: a 1.6 x 0.8 x 0.8 m channel at d = 0.002 m.
+
+    Three wall layers, open top (3.9M wall particles); a fluid column 0.2 x 0.8
+    x 0.4 m at the x = 0 end (100 x 400 x 200 = 8.0M particles) released into
+    the channel; ``nbodies`` debris cuboids with sides of U{4..12} particles per
+    axis (seeded, independently) and densities U[500, 900] kg/m^3, one per slot
+    of a 15 d grid over x in [0.3, 1.6) m in two layers (z = 0 and 15 d), the
+    slots drawn by a seeded permutation, each body at a seeded lattice offset
+    inside its slot.  ``scale`` < 1 shrinks every length (same d) and the slot
+    grid with it for quick tests.
+    """
+    d = 0.002
+    sc = scale
+    interior = (1.6 * sc, 0.8 * sc, 0.8 * sc)
+    fluid = (0.2 * sc, 0.8 * sc, 0.4 * sc)
+    pitch = 15 * d
+    x0 = 0.3 * sc
+    nxs = int((interior[0] - x0) // pitch)
+    nys = int(interior[1] // pitch)
+    slots = [(ix, iy, iz) for iz in range(2) for ix in range(nxs) for iy in range(nys)]
+    if nbodies > len(slots):
+        raise ValueError("channel_c5: %d bodies, %d slots" % (nbodies, len(slots)))
+    order = np.argsort(uniform(seed, len(slots), stream=9))[:nbodies]
+    size = (4 + np.floor(uniform(seed, 3 * nbodies, stream=10, lo=0.0, hi=9.0 - 1e-9))).astype(int).reshape(nbodies, 3)
+    off = uniform(seed, 2 * nbodies, stream=11).reshape(nbodies, 2)
+    rho = uniform(seed, nbodies, stream=12, lo=500.0, hi=900.0)
+    deb = []
+    for b, k in enumerate(order):
+        ix, iy, iz = slots[k]
+        free = 15 - size[b, :2]   # lattice steps left in the slot
+        ox = x0 + (ix * 15 + int(off[b, 0] * (free[0] + 1) * 0.999)) * d
+        oy = (iy * 15 + int(off[b, 1] * (free[1] + 1) * 0.999)) * d
+        deb.append(((ox, oy, iz * pitch), tuple(int(v) for v in size[b]), float(rho[b])))
+    return dam_break(d, interior, fluid, layers=3, debris=deb, dim=3, seed=seed, rigid=rigid)
+
+
 def small_2d(seed: int = 0, debris_velocity=(0.3, -0.2, 0.0), fluid_velocity=(0.0, 0.0, 0.0),
              nfx: int = 16, nfy: int = 12, d: float = 0.002, gate: bool = True) -> Scene:
     """Small 2D tank (one particle layer in z), shaped like the reference's shipped case.
@@ -416,6 +463,13 @@ def params_c1(**over) -> Dict[str, object]:
 def params_c3(**over) -> Dict[str, object]:
     """C3 parameters: as C1 at d = 0.01 m, dt = DEMDt = 5e-5 s (the same acoustic CFL as C1 and C2)."""
     p = params_c1(Dt=5.0e-5, DEMDt=5.0e-5, EndTime=0.15, OutputInterval=1.0, VtkOutputInterval=1.0)
+    p.update(over)
+    return p
+
+
+def params_c5(**over) -> Dict[str, object]:
+    """C5 parameters: as C1 at d = 0.002 m, dt = DEMDt = 1e-5 s (the acoustic CFL of C2), 1000 steps."""
+    p = params_c1(Dt=1.0e-5, DEMDt=1.0e-5, EndTime=0.01, OutputInterval=1.0, VtkOutputInterval=1.0)
     p.update(over)
     return p
 

@@ -147,6 +147,16 @@ class Surrogate:
         self._bbox = None
         self._folded = None   # inference: per round [n1 left | n1 right @ w2] (rebuilt after a weight change)
 
+    def at_spacing(self, d: float) -> "Surrogate":
+        """Use the network on particles of spacing d: the graph radius becomes
+        3 d.  Every input is in units of the radius or of the velocity scale,
+        so the shapes fit; the accuracy at a spacing other than the training
+        one is not established (used for throughput on other scenes)."""
+        self.d = d
+        self.radius = 3.0 * d
+        self._bbox = None
+        return self
+
     # ------------------------------------------------------------------ parameters
     def _refresh_bf16(self) -> None:
         """bf16 copies of every GEMM weight (an Adam step with lr 0 rewrites them)."""
@@ -232,8 +242,10 @@ class Surrogate:
                                    _p(ws["counts"]), _p(ws["start"]), _p(ws["fill"]), _p(ws["order"]),
                                    _p(ws["sorted"]), _p(ws["deg"]), _p(rp), _p(ci), _p(ws["eps"] if eps else None),
                                    ws["cap"], C.byref(nnz), _p(ws["scan"]), stream)
+            if rc == -3:
+                raise RuntimeError("radius graph: more than 2^31 edges (radius %.3g m for %d nodes)" % (self.radius, N))
             if rc == -2:
-                ws["cap"] = int(nnz.value * 1.25) + 1024
+                ws["cap"] = min(int(nnz.value * 1.25) + 1024, 2 ** 31 - 1)
                 ws["ci"] = torch.empty(ws["cap"], dtype=torch.int32, device=self.dev)
                 ws["eps"] = None
                 continue

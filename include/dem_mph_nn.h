@@ -79,7 +79,8 @@ int dmph_nn_edge_bwd(const uint16_t *AB, const float *pos, const int *rp, const 
                      float *scratch, int accumulate, void *stream);
 
 /* radius graph (CSR rp[N + 1], ci[cap]) of the nodes at pos [N][3] (scene ids or NULL):
- * rows ascending by cell then node; *nnz_out = edges; returns -2 if cap < nnz (call again).
+ * rows ascending by cell then node; *nnz_out = edges; returns -2 if cap < nnz (call again),
+ * -3 (nothing written, *nnz_out = -1) if the edges would not fit int32 row offsets.
  * eps (NULL or [2 cap] words): also each edge's features as edge_eps writes them.
  * Work arrays: key, order, deg [N], sorted [4N] (cell-sorted positions and ids);
  * counts, start, fill [cells * scenes + 1]; scratch of

@@ -143,6 +143,19 @@ def test_radius_graph_matches_brute_force(S):
     assert nnz > 10 * len(pos)
 
 
+def test_radius_graph_refuses_more_than_int32_edges(S):
+    """50,000 nodes inside one radius: 2.5e9 edges do not fit int32 row
+    offsets; the build stops before writing any row and says so."""
+    rng = np.random.default_rng(9)
+    pos = torch.tensor(rng.uniform(0.0, 0.01, (50_000, 3)).astype(np.float32), device="cuda:0")
+    sur = S.Surrogate(d=1.0, tank_lo=[0, 0, 0], tank_hi=[1, 1, 1], vscale=1.0, ascale=1.0)
+    with pytest.raises(RuntimeError, match="2\\^31"):
+        sur.graph(pos)
+    # the same nodes at a small radius still build
+    rp, ci, nnz = sur.at_spacing(0.0005).graph(pos)
+    assert 0 < nnz == int(rp[-1].item())
+
+
 def _small_problem(S, seed=0):
     from dem_mph_amd import scenes as SC
     sc = SC.ensemble_c3(1, seed=seed, d=0.02, interior=(0.4, 0.2, 0.3), fluid_range=(200.0, 600.0))[0]

@@ -13,6 +13,56 @@ def test_c2_counts():
     assert 500.0 <= rho.min() and rho.max() <= 900.0
 
 
+def test_tank_walls_are_the_padded_lattice_without_its_interior():
+    """The plane-by-plane wall builder against the definition, brute force."""
+    for interior, d, closed, dim in (((0.2, 0.1, 0.15), 0.01, False, 3), ((0.2, 0.1, 0.15), 0.01, True, 3),
+                                     ((0.3, 0.2, 0.01), 0.01, False, 2)):
+        L = np.asarray(interior)
+        lo = -3 * d * np.ones(3)
+        hi = L + 3 * d
+        if not closed:
+            hi[2 if dim == 3 else 1] = L[2 if dim == 3 else 1]
+        if dim == 2:
+            lo[2], hi[2] = 0.0, d
+        full = S.lattice(lo, hi, d, dim)
+        inside = np.all([(full[:, a] > 0) & (full[:, a] < L[a]) for a in range(dim)], axis=0)
+        np.testing.assert_array_equal(S.tank_walls(interior, d, 3, closed, dim), full[~inside])
+
+
+def test_c5_counts_at_full_size():
+    """8.0M fluid (100 x 400 x 200), 2048 bodies of 4-12 per side (~1.05M), ~3.9M
+    walls (open top); bodies inside the channel, clear of the fluid column and of
+    one another."""
+    sc = S.channel_c5(seed=1)
+    c = sc.counts()
+    assert c["fluid"] == 8_000_000 and c["bodies"] == 2048
+    assert 0.95e6 < c["solid"] < 1.15e6 and 3.8e6 < c["wall"] < 4.0e6
+    sizes = np.bincount(sc.body[sc.body >= 0])
+    assert sizes.min() >= 64 and sizes.max() <= 1728
+    deb = sc.pos[sc.type == S.SOLID]
+    assert deb[:, 0].min() > 0.3 - 0.01 * sc.d and deb[:, 0].max() < 1.6 and deb[:, 1].max() < 0.8
+    assert deb[:, 2].min() > 0.0
+    rho = sc.density[sc.type == S.SOLID]
+    assert 500.0 <= rho.min() and rho.max() <= 900.0
+
+
+def test_c5_small_scale_is_seeded_and_separated():
+    from scipy.spatial import cKDTree
+    a = S.channel_c5(seed=3, scale=0.25, nbodies=40)
+    b = S.channel_c5(seed=3, scale=0.25, nbodies=40)
+    np.testing.assert_array_equal(a.pos, b.pos)
+    assert a.counts()["bodies"] == 40 and a.counts()["fluid"] == 25 * 100 * 50
+    solid = a.pos[a.type == S.SOLID]
+    other = a.pos[a.type != S.SOLID]
+    # no debris particle within half a spacing of any other particle of another body or of the rest
+    assert len(cKDTree(solid).query_ball_tree(cKDTree(other), 0.5 * a.d)[0]) == 0
+    body = a.body[a.type == S.SOLID]
+    pairs = cKDTree(solid).query_pairs(0.99 * a.d, output_type="ndarray")
+    assert (body[pairs[:, 0]] == body[pairs[:, 1]]).all()
+    with pytest.raises(ValueError):
+        S.channel_c5(seed=3, scale=0.25, nbodies=10_000)
+
+
 @pytest.fixture(scope="module")
 def ensemble():
     return S.ensemble_c3(512, seed=0)
