@@ -42,13 +42,15 @@ def S():
 
 
 @pytest.mark.parametrize("kstep", [16, 32])
-@pytest.mark.parametrize("K,N,relu,res,half", [(32, 128, 1, 0, 0), (128, 128, 0, 1, 0), (256, 128, 1, 0, 0),
-                                               (128, 256, 0, 0, 0), (128, 256, 0, 0, 1), (256, 128, 1, 1, 1)])
-def test_gemm_matches_numpy(S, kstep, K, N, relu, res, half):
+@pytest.mark.parametrize("K,N,relu,res,half,M", [(32, 128, 1, 0, 0, 300), (128, 128, 0, 1, 0, 300),
+                                                 (256, 128, 1, 0, 0, 300), (128, 256, 0, 0, 0, 300),
+                                                 (128, 256, 0, 0, 1, 300), (256, 128, 1, 1, 1, 300),
+                                                 (128, 128, 0, 1, 0, 200_001), (256, 128, 1, 0, 0, 200_001)])
+def test_gemm_matches_numpy(S, kstep, K, N, relu, res, half, M):
     """bf16 (or IEEE half, flag DMPH_NN_F16_OUT) output against numpy on the
-    same bf16 inputs; M = 300 leaves a partial row tile."""
+    same bf16 inputs; M = 300 leaves a partial row tile, M = 200,001 makes the
+    persistent workgroups walk several tiles each, the last one partial."""
     rng = np.random.default_rng(K + N + kstep)
-    M = 300
     X = rng.standard_normal((M, K)).astype(np.float32)
     Wt = (rng.standard_normal((N, K)) / np.sqrt(K)).astype(np.float32)
     b = rng.standard_normal(N).astype(np.float32)
@@ -73,12 +75,13 @@ def test_gemm_matches_numpy(S, kstep, K, N, relu, res, half):
 
 
 @pytest.mark.parametrize("kstep", [16, 32])
-def test_gemm_head_matches_gemm_then_head(S, kstep):
+@pytest.mark.parametrize("M", [301, 200_003])
+def test_gemm_head_matches_gemm_then_head(S, kstep, M):
     """Inference decoder tail (hidden layer + 3-output head in one kernel)
     against the two kernels the training path runs, on the same bf16 inputs;
-    M = 301 leaves a partial tile; X with a row stride of 256."""
+    partial last tiles; at M = 200,003 each persistent workgroup walks several
+    tiles; X with a row stride of 256."""
     rng = np.random.default_rng(kstep)
-    M = 301
     X = rng.standard_normal((M, 256)).astype(np.float32)
     Wt = (rng.standard_normal((128, 128)) / np.sqrt(128)).astype(np.float32)
     b = rng.standard_normal(128).astype(np.float32)
