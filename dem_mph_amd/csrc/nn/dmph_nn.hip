@@ -430,7 +430,8 @@ __global__ void __launch_bounds__(256) k_edge_eps(const float *__restrict__ pos,
 __global__ void __launch_bounds__(256) k_edge_fwd_h(const h2_t *__restrict__ AB, const int *__restrict__ rp,
                                                     const int *__restrict__ ci, const uint2 *__restrict__ eps,
                                                     const float *__restrict__ Wc, const float *__restrict__ bc,
-                                                    bf16_t *__restrict__ S, int lds, int N) {
+                                                    bf16_t *__restrict__ S, int lds, int N,
+                                                    const int *__restrict__ deg, int row_cap) {
     const int i = __builtin_amdgcn_readfirstlane(xcd_block(gridDim.x) * 4 + (threadIdx.x >> 6));
     if (i >= N) return;
     const int lane = threadIdx.x & 63, f = 2 * lane;
@@ -448,8 +449,8 @@ __global__ void __launch_bounds__(256) k_edge_fwd_h(const h2_t *__restrict__ AB,
         return *reinterpret_cast<const h2_t *>(row + lane_bytes);
     };
     float s0 = 0.f, s1 = 0.f;
-    const int q1 = rp[i + 1];
-    int q = rp[i];
+    int q = deg ? i * row_cap : rp[i];   // rows of a fixed capacity (row_cap entries, deg used) or CSR
+    const int q1 = deg ? q + deg[i] : rp[i + 1];
     auto edge = [&](h2_t bj, uint2 e, h2_t acc) {
         const h2_t exy = bitsh2(e.x), ezd = bitsh2(e.y);
         h2_t c = a2 + bj;
@@ -687,12 +688,14 @@ __global__ void k_g_rank(const int *key, const int *start, const int *order, con
 // 64 candidates at a time and the hits of a step are stored contiguously
 // (ballot + prefix), so a row is written in whole runs.  FILL writes ci (and,
 // when eps is given, the edge features of edge_eps), else the degree.
-template <bool FILL>
+// SLAB (FILL only): row i starts at i * row_cap instead of rp[i]; entries past
+// row_cap are not written, the degree is, and *over counts the rows cut.
+template <bool FILL, bool SLAB = false>
 __global__ void __launch_bounds__(256) k_g_rows(const float *__restrict__ pos, const int *__restrict__ key,
                                                 const int *__restrict__ start, const float4 *__restrict__ ps, int N,
                                                 NnGrid gd, float r2, float inv_r, int *__restrict__ deg,
                                                 const int *__restrict__ rp, int *__restrict__ ci,
-                                                uint2 *__restrict__ eps) {
+                                                uint2 *__restrict__ eps, int row_cap = 0, int *over = nullptr) {
     const int i = __builtin_amdgcn_readfirstlane(xcd_block(gridDim.x) * 4 + (threadIdx.x >> 6));
     if (i >= N) return;
     const int lane = threadIdx.x & 63;
@@ -702,7 +705,8 @@ __global__ void __launch_bounds__(256) k_g_rows(const float *__restrict__ pos, c
     const int c[3] = {cell / (gd.cc[1] * gd.cc[2]), (cell / gd.cc[2]) % gd.cc[1], cell % gd.cc[2]};
     const float xi = pos[3 * i], yi = pos[3 * i + 1], zi = pos[3 * i + 2];
     const int z0 = max(c[2] - 1, 0), z1 = min(c[2] + 1, gd.cc[2] - 1);
-    int out = FILL ? rp[i] : 0;
+    const int row0 = SLAB ? 0 : (FILL ? rp[i] : 0);
+    int out = row0;
     for (int x = max(c[0] - 1, 0); x <= min(c[0] + 1, gd.cc[0] - 1); x++)
         for (int y = max(c[1] - 1, 0); y <= min(c[1] + 1, gd.cc[1] - 1); y++) {
             const int col = sbase + (x * gd.cc[1] + y) * gd.cc[2];
@@ -722,17 +726,23 @@ __global__ void __launch_bounds__(256) k_g_rows(const float *__restrict__ pos, c
                 if (FILL && hit) {
                     const int o = out + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
                                                                      __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-                    ci[o] = j;
-                    if (eps) {
-                        const float d = sqrtf(dx * dx + dy * dy + dz * dz);
-                        eps[o] = make_uint2(h2bits(h2_t{(_Float16)(dx * inv_r), (_Float16)(dy * inv_r)}),
-                                            h2bits(h2_t{(_Float16)(dz * inv_r), (_Float16)(d * inv_r)}));
+                    if (!SLAB || o < row_cap) {   // a cut row is reported, never written past
+                        const size_t at = SLAB ? (size_t)i * row_cap + o : (size_t)o;
+                        ci[at] = j;
+                        if (eps) {
+                            const float d = sqrtf(dx * dx + dy * dy + dz * dz);
+                            eps[at] = make_uint2(h2bits(h2_t{(_Float16)(dx * inv_r), (_Float16)(dy * inv_r)}),
+                                                 h2bits(h2_t{(_Float16)(dz * inv_r), (_Float16)(d * inv_r)}));
+                        }
                     }
                 }
                 out += __popcll(m);
             }
         }
-    if (!FILL && lane == 0) deg[i] = out;
+    if ((!FILL || SLAB) && lane == 0) {
+        deg[i] = out - row0;
+        if (SLAB && out > row_cap) atomicAdd(over, 1);
+    }
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -1021,12 +1031,12 @@ int dmph_nn_edge_eps(const float *pos, const int *rp, const int *ci, uint32_t *e
 }
 
 int dmph_nn_edge_fwd_f16(const uint16_t *AB, const int *rp, const int *ci, const uint32_t *eps, const float *Wc,
-                         const float *bc, uint16_t *S, int lds, int N, void *stream) {
+                         const float *bc, uint16_t *S, int lds, int N, const int *deg, int row_cap, void *stream) {
     if (N <= 0) return 0;
-    if (lds < 128 || lds % 2) return -1;
+    if (lds < 128 || lds % 2 || (deg && (row_cap <= 0 || (long long)N * row_cap > 0x7fffffffLL))) return -1;
     hipLaunchKernelGGL(k_edge_fwd_h, dim3((N + 3) / 4), dim3(256), 0, (hipStream_t)stream,
                        reinterpret_cast<const h2_t *>(AB), rp, ci, reinterpret_cast<const uint2 *>(eps), Wc, bc, S, lds,
-                       N);
+                       N, deg, row_cap);
     return launch_ok();
 }
 
@@ -1056,10 +1066,11 @@ static void scan_excl(const int *in, int *out, int n, int *scratch, hipStream_t 
     hipLaunchKernelGGL(k_scan_tiles, dim3(nt), dim3(256), 0, s, in, out, n, scratch + nt + 1);
 }
 
-// scratch layout: [0, 2) ints the 64-bit edge total, then the scan carries
+// scratch layout: [0, 2) ints the 64-bit edge total (dmph_nn_graph) or [2]
+// the cut-row count (dmph_nn_graph_rows), then the scan carries from [4]
 size_t dmph_nn_graph_scratch(int N, int ncells_total) {
     const int n = N > ncells_total ? N : ncells_total;
-    return (size_t)(2 * ((n + kTile - 1) / kTile) + 4) * sizeof(int);
+    return (size_t)(2 * ((n + kTile - 1) / kTile) + 6) * sizeof(int);
 }
 
 size_t dmph_nn_bbox_scratch(int N) { (void)N; return 256 * 6 * sizeof(float); }
@@ -1083,7 +1094,7 @@ int dmph_nn_graph(const float *pos, const int *scene, int N, const NnGrid *grid,
     const int b = (N + 255) / 256;
     hipLaunchKernelGGL(k_g_keys, dim3(b), dim3(256), 0, s, pos, scene, N, *grid, key, counts);
     unsigned long long *total = reinterpret_cast<unsigned long long *>(scratch);
-    int *carries = scratch + 2;
+    int *carries = scratch + 4;
     if (hipMemsetAsync(total, 0, sizeof(unsigned long long), s) != hipSuccess) return -5;
     scan_excl(counts, start, nk, carries, s);
     hipLaunchKernelGGL(k_g_scatter, dim3(b), dim3(256), 0, s, key, start, fill, order, N);
@@ -1109,6 +1120,31 @@ int dmph_nn_graph(const float *pos, const int *scene, int N, const NnGrid *grid,
     hipLaunchKernelGGL(k_g_rows<true>, dim3(bw), dim3(256), 0, s, pos, key, start, ps, N, *grid, r2, 1.f / radius,
                        deg, rp, ci, reinterpret_cast<uint2 *>(eps));
     return launch_ok();
+}
+
+int dmph_nn_graph_rows(const float *pos, const int *scene, int N, const NnGrid *grid, float radius, int *key,
+                       int *counts, int *start, int *fill, int *order, int *sorted, int *deg, int *ci, uint32_t *eps,
+                       int row_cap, int *over_out, int *scratch, void *stream) {
+    if (row_cap <= 0) return -1;
+    hipStream_t s = (hipStream_t)stream;
+    const int nk = grid->ncell * (grid->nscene > 0 ? grid->nscene : 1);
+    int *over = scratch + 2;   // graph scratch: [2, 3) the cut-row count here
+    if (hipMemsetAsync(counts, 0, (size_t)(nk + 1) * sizeof(int), s) != hipSuccess) return -5;
+    if (hipMemsetAsync(fill, 0, (size_t)(nk + 1) * sizeof(int), s) != hipSuccess) return -5;
+    if (hipMemsetAsync(over, 0, sizeof(int), s) != hipSuccess) return -5;
+    const int b = (N + 255) / 256;
+    hipLaunchKernelGGL(k_g_keys, dim3(b), dim3(256), 0, s, pos, scene, N, *grid, key, counts);
+    scan_excl(counts, start, nk, scratch + 4, s);
+    hipLaunchKernelGGL(k_g_scatter, dim3(b), dim3(256), 0, s, key, start, fill, order, N);
+    float4 *ps = reinterpret_cast<float4 *>(sorted);
+    hipLaunchKernelGGL(k_g_rank, dim3(b), dim3(256), 0, s, key, start, order, pos, ps, N);
+    hipLaunchKernelGGL((k_g_rows<true, true>), dim3((N + 3) / 4), dim3(256), 0, s, pos, key, start, ps, N, *grid,
+                       radius * radius, 1.f / radius, deg, nullptr, ci, reinterpret_cast<uint2 *>(eps), row_cap, over);
+    int cut = 0;
+    if (hipMemcpyAsync(&cut, over, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess) return -5;
+    if (hipStreamSynchronize(s) != hipSuccess) return -5;
+    *over_out = cut;
+    return cut ? -2 : launch_ok();
 }
 
 int dmph_nn_features(const float *vel_hist, int N, const int *cls, const float *pos, const NnFeat *ft, uint16_t *X,

@@ -209,29 +209,8 @@ class Surrogate:
         N = pos.shape[0]
         lib = nnlib()
         stream = C.c_void_p(torch.cuda.current_stream(self.dev).cuda_stream)
-        if self._bbox is None:
-            self._bbox = (torch.empty(6, dtype=torch.float32, device=self.dev),
-                          torch.empty(lib.dmph_nn_bbox_scratch(N) // 4 + 1, dtype=torch.float32, device=self.dev))
-        _chk(lib.dmph_nn_bbox(_p(pos), N, _p(self._bbox[0]), _p(self._bbox[1]), stream), "dmph_nn_bbox")
-        box = self._bbox[0].cpu().numpy().astype(np.float64)
-        lo, hi = box[:3] - 1e-6, box[3:] + 1e-6
-        w = self.radius * (1.0 + 1e-5)            # cubic cells at least the radius wide
-        cc = np.maximum(np.ceil((hi - lo) / w).astype(int), 1)
-        g = NnGrid()
-        for k in range(3):
-            g.lo[k] = float(lo[k])
-            g.cc[k] = int(cc[k])
-        g.inv_w = float(1.0 / w)
-        g.ncell = int(cc.prod())
-        g.nscene = nscene
-        nk = g.ncell * nscene
-        ws = self._graph_ws
-        if ws is None or ws["N"] < N or ws["nk"] < nk:
-            i32 = lambda n: torch.empty(max(n, 1), dtype=torch.int32, device=self.dev)
-            ws = {"N": N, "nk": nk, "key": i32(N), "order": i32(N), "sorted": i32(4 * N), "deg": i32(N),
-                  "counts": i32(nk + 1), "start": i32(nk + 1), "fill": i32(nk + 1), "cap": 0, "ci": None, "eps": None,
-                  "scan": i32(lib.dmph_nn_graph_scratch(N, nk) // 4 + 1)}
-            self._graph_ws = ws
+        g, nk = self._grid(pos, nscene, stream)
+        ws = self._graph_work(N, nk)
         rp = torch.empty(N + 1, dtype=torch.int32, device=self.dev)
         nnz = C.c_int(0)
         for _ in range(2):
@@ -253,6 +232,69 @@ class Surrogate:
             break
         self.graph_eps = ws["eps"] if eps else None
         return rp, ws["ci"], int(nnz.value)
+
+    def _grid(self, pos, nscene: int, stream):
+        """Uniform grid over the nodes' bounding box, cells at least the radius wide."""
+        torch = self.torch
+        N = pos.shape[0]
+        lib = nnlib()
+        if self._bbox is None:
+            self._bbox = (torch.empty(6, dtype=torch.float32, device=self.dev),
+                          torch.empty(lib.dmph_nn_bbox_scratch(N) // 4 + 1, dtype=torch.float32, device=self.dev))
+        _chk(lib.dmph_nn_bbox(_p(pos), N, _p(self._bbox[0]), _p(self._bbox[1]), stream), "dmph_nn_bbox")
+        box = self._bbox[0].cpu().numpy().astype(np.float64)
+        lo, hi = box[:3] - 1e-6, box[3:] + 1e-6
+        w = self.radius * (1.0 + 1e-5)            # cubic cells at least the radius wide
+        cc = np.maximum(np.ceil((hi - lo) / w).astype(int), 1)
+        g = NnGrid()
+        for k in range(3):
+            g.lo[k] = float(lo[k])
+            g.cc[k] = int(cc[k])
+        g.inv_w = float(1.0 / w)
+        g.ncell = int(cc.prod())
+        g.nscene = nscene
+        return g, g.ncell * nscene
+
+    def _graph_work(self, N: int, nk: int):
+        ws = self._graph_ws
+        if ws is None or ws["N"] < N or ws["nk"] < nk:
+            torch = self.torch
+            i32 = lambda n: torch.empty(max(n, 1), dtype=torch.int32, device=self.dev)
+            ws = {"N": N, "nk": nk, "key": i32(N), "order": i32(N), "sorted": i32(4 * N), "deg": i32(N),
+                  "counts": i32(nk + 1), "start": i32(nk + 1), "fill": i32(nk + 1), "cap": 0, "ci": None, "eps": None,
+                  "scan": i32(nnlib().dmph_nn_graph_scratch(N, nk) // 4 + 1)}
+            self._graph_ws = ws
+        return ws
+
+    def graph_rows(self, pos, scene=None, nscene: int = 1, row_cap: int = 160):
+        """Inference graph in one pass: rows of a fixed capacity (row i at
+        i * row_cap, deg[i] entries) with the edge features; the capacity grows
+        while rows are cut.  Returns (ci, deg, row_cap, eps), or None where N x
+        row_cap passes int32 offsets (then use graph())."""
+        torch = self.torch
+        N = pos.shape[0]
+        lib = nnlib()
+        stream = C.c_void_p(torch.cuda.current_stream(self.dev).cuda_stream)
+        g, nk = self._grid(pos, nscene, stream)
+        ws = self._graph_work(N, nk)
+        cap = max(ws.get("row_cap", 0), row_cap)
+        over = C.c_int(0)
+        while True:
+            if N * cap > 2 ** 31 - 1:
+                return None
+            if ws.get("row_cap", 0) != cap or ws.get("rows_n", 0) < N:
+                ws["rows_ci"] = torch.empty(N * cap, dtype=torch.int32, device=self.dev)
+                ws["rows_eps"] = torch.empty(2 * N * cap, dtype=torch.int32, device=self.dev)
+                ws["row_cap"], ws["rows_n"] = cap, N
+            rc = lib.dmph_nn_graph_rows(_p(pos), _p(scene), N, C.byref(g), C.c_float(self.radius), _p(ws["key"]),
+                                        _p(ws["counts"]), _p(ws["start"]), _p(ws["fill"]), _p(ws["order"]),
+                                        _p(ws["sorted"]), _p(ws["deg"]), _p(ws["rows_ci"]), _p(ws["rows_eps"]), cap,
+                                        C.byref(over), _p(ws["scan"]), stream)
+            if rc == -2:
+                cap = (cap * 3 // 2 + 15) // 16 * 16
+                continue
+            _chk(rc, "dmph_nn_graph_rows")
+            return ws["rows_ci"], ws["deg"], cap, ws["rows_eps"]
 
     # ------------------------------------------------------------------ forward / backward
     def _workspace(self, N: int, train: bool = True):
@@ -338,7 +380,7 @@ class Surrogate:
             self._folded = out
         return self._folded
 
-    def forward(self, x0, pos, rp, ci, N: int, train: bool = True, eps=None):
+    def forward(self, x0, pos, rp, ci, N: int, train: bool = True, eps=None, rows=None):
         """Accelerations (scaled) y [N, 3] FP32; with `train` every round's
         activations are kept for backward, else two slots alternate and the
         aggregation's linear layer is folded into the node MLP (_fold)."""
@@ -368,8 +410,9 @@ class Surrogate:
                 self._gemm(sr, W, L["w2"], hin[:, W:], 2 * W, N, False)
                 self._gemm(hin, 2 * W, L["n1"], u, W, N, True)
             else:   # S straight into the right half of hin_r, then the folded node layer
+                deg, cap = rows if rows is not None else (None, 0)
                 _chk(lib.dmph_nn_edge_fwd_f16(_p(ab), _p(rp), _p(ci), _p(eps), _p(L["c"].w), _p(L["c"].b),
-                                              _p(hin[:, W:]), 2 * W, N, s), "edge_fwd_f16")
+                                              _p(hin[:, W:]), 2 * W, N, _p(deg), cap, s), "edge_fwd_f16")
                 self._gemm(hin, 2 * W, folded[r], u, W, N, True)
             self._gemm(u, W, L["n2"], ws["hin"][hslot(r + 1)], 2 * W, N, False, R=hin, ldr=2 * W)
         if train:
@@ -438,6 +481,18 @@ class Surrogate:
                 _chk(lib.dmph_nn_adam(_p(l.b), _p(l.gb), _p(l.mb), _p(l.vb), l.b.numel(), C.c_float(lr),
                                       C.c_float(b1), C.c_float(b2), C.c_float(eps), self.adam_step, C.c_float(0.0),
                                       None, None, 1, l.b.numel(), s), "adam")
+
+    def infer(self, pos, vel_hist, cls, scene=None, nscene: int = 1):
+        """One inference pass: graph (fixed-capacity rows where N x capacity fits
+        int32 offsets, else CSR), features, network; returns y [N, 3]."""
+        N = pos.shape[0]
+        r = self.graph_rows(pos, scene, nscene)
+        x0 = self.features(vel_hist, cls, pos, train=False)
+        if r is not None:
+            ci, deg, cap, eps = r
+            return self.forward(x0, pos, None, ci, N, train=False, eps=eps, rows=(deg, cap))
+        rp, ci, _ = self.graph(pos, scene, nscene, eps=True)
+        return self.forward(x0, pos, rp, ci, N, train=False, eps=self.graph_eps)
 
     def rollout_step(self, pos, vel_hist, y, cls, body, nbodies: int, h: float, body_sums) -> None:
         """x, v history advanced by stride h from the network's output y."""
@@ -607,9 +662,7 @@ def rollout(sur: Optional[Surrogate], traj: Trajectories, start: int, steps: int
     for k in range(1, steps + 1):
         t0 = _now()
         if baseline is None:
-            rp, ci, _ = sur.graph(pos, traj.scene, traj.nscene, eps=True)
-            x0 = sur.features(vh, traj.cls, pos, train=False)
-            y = sur.forward(x0, pos, rp, ci, n, train=False, eps=sur.graph_eps)
+            y = sur.infer(pos, vh, traj.cls, traj.scene, traj.nscene)
             ascale = sur.ascale
         else:
             a = np.zeros(3) if baseline == "still" else g
@@ -667,9 +720,7 @@ def advance_batch(sur: Surrogate, scenes, steps: int, dt: float) -> Dict[str, fl
     h = STRIDE * dt
 
     def one():
-        rp, ci, _ = sur.graph(pos, scene, len(scenes), eps=True)
-        x0 = sur.features(vh, cls, pos, train=False)
-        y = sur.forward(x0, pos, rp, ci, n, train=False, eps=sur.graph_eps)
+        y = sur.infer(pos, vh, cls, scene, len(scenes))
         sur.rollout_step(pos, vh, y, cls, body, boff, h, body_sums)
 
     one()   # warm-up (workspace, graph capacity)

@@ -67,10 +67,12 @@ int dmph_nn_edge_fwd(const uint16_t *AB, const float *pos, const int *rp, const 
  * graph as two IEEE-half pairs, eps[q] = ((dx, dy), (dz, |d|)) / radius (row
  * order of ci; eps holds nnz entries); edge_fwd_f16 then sums with AB in IEEE
  * half ([N][256], DMPH_NN_F16_OUT), packed half arithmetic in batches of 16
- * edges added into FP32 sums; S as in edge_fwd (bf16). */
+ * edges added into FP32 sums; S as in edge_fwd (bf16).  With deg (not NULL) the
+ * rows are those of dmph_nn_graph_rows: row i at i * row_cap, deg[i] entries
+ * (rp unused), edge_eps not needed (the graph writes eps). */
 int dmph_nn_edge_eps(const float *pos, const int *rp, const int *ci, uint32_t *eps, int N, float radius, void *stream);
 int dmph_nn_edge_fwd_f16(const uint16_t *AB, const int *rp, const int *ci, const uint32_t *eps, const float *Wc,
-                         const float *bc, uint16_t *S, int lds, int N, void *stream);
+                         const float *bc, uint16_t *S, int lds, int N, const int *deg, int row_cap, void *stream);
 /* gradients of edge_fwd given dS [N][128] FP32: dAB [N][256] bf16, dWc, dbc (+)=;
  * the graph must be symmetric; scratch of dmph_nn_edge_scratch bytes */
 size_t dmph_nn_edge_scratch(int N);
@@ -89,6 +91,15 @@ size_t dmph_nn_graph_scratch(int N, int ncells_total);
 int dmph_nn_graph(const float *pos, const int *scene, int N, const NnGrid *grid, float radius, int *key, int *counts,
                   int *start, int *fill, int *order, int *sorted, int *deg, int *rp, int *ci, uint32_t *eps, int cap,
                   int *nnz_out, int *scratch, void *stream);
+
+/* the same graph in rows of a fixed capacity, one pass (no degree count first):
+ * row i at ci + i * row_cap with deg[i] entries, same order as dmph_nn_graph;
+ * eps (NULL or [2 N row_cap] words) as edge_eps writes it.  Returns -2 with
+ * *over_out = rows longer than row_cap (their tails unwritten: call again with a
+ * larger row_cap or use dmph_nn_graph); same work arrays and scratch. */
+int dmph_nn_graph_rows(const float *pos, const int *scene, int N, const NnGrid *grid, float radius, int *key,
+                       int *counts, int *start, int *fill, int *order, int *sorted, int *deg, int *ci, uint32_t *eps,
+                       int row_cap, int *over_out, int *scratch, void *stream);
 
 /* bounding box of pos [N][3] into out[6] (min x y z, max x y z; device memory);
  * scratch of dmph_nn_bbox_scratch(N) bytes */

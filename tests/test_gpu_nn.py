@@ -146,6 +146,38 @@ def test_radius_graph_matches_brute_force(S):
     assert nnz > 10 * len(pos)
 
 
+def test_fixed_capacity_rows_equal_the_csr_graph(S):
+    """dmph_nn_graph_rows: the rows of dmph_nn_graph, entry for entry, at
+    i * row_cap; started at 16 entries the capacity grows past the longest row
+    (the cut rows are reported, nothing is written past a row's end); the edge
+    sums over the rows equal those over the CSR graph."""
+    sur, sc, pos, vh, cls, tgt, mask = _small_problem(S, seed=4)
+    n = sc.n
+    rp, ci, nnz = sur.graph(pos, eps=True)
+    RP, CI = rp.cpu().numpy(), ci.cpu().numpy()
+    E = sur.graph_eps[:2 * nnz].cpu().numpy().reshape(-1, 2)
+    ci2, deg, cap, eps2 = sur.graph_rows(pos, row_cap=16)
+    assert cap >= np.diff(RP).max() and cap % 16 == 0
+    D, C2, E2 = deg[:n].cpu().numpy(), ci2.cpu().numpy().reshape(n, cap), eps2.cpu().numpy().reshape(n, cap, 2)
+    np.testing.assert_array_equal(D, np.diff(RP))
+    for i in range(n):
+        np.testing.assert_array_equal(C2[i, :D[i]], CI[RP[i]:RP[i + 1]])
+        np.testing.assert_array_equal(E2[i, :D[i]], E[RP[i]:RP[i + 1]])
+    rng = np.random.default_rng(4)
+    abd = torch.tensor((rng.standard_normal((n, 256)) * 0.5).astype(np.float16).view(np.int16), device="cuda:0")
+    Wc = torch.tensor((rng.standard_normal((128, 4)) * 0.5).astype(np.float32), device="cuda:0")
+    bc = torch.tensor((rng.standard_normal(128) * 0.2).astype(np.float32), device="cuda:0")
+    lib = S.nnlib()
+    s = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    o1 = torch.zeros((n, 128), dtype=torch.int16, device="cuda:0")
+    o2 = torch.zeros((n, 128), dtype=torch.int16, device="cuda:0")
+    assert lib.dmph_nn_edge_fwd_f16(_p(abd), _p(rp), _p(ci), _p(sur.graph_eps), _p(Wc), _p(bc), _p(o1), 128, n,
+                                    None, 0, s) == 0
+    assert lib.dmph_nn_edge_fwd_f16(_p(abd), None, _p(ci2), _p(eps2), _p(Wc), _p(bc), _p(o2), 128, n, _p(deg), cap,
+                                    s) == 0
+    assert torch.equal(o1, o2)
+
+
 def test_radius_graph_refuses_more_than_int32_edges(S):
     """50,000 nodes inside one radius: 2.5e9 edges do not fit int32 row
     offsets; the build stops before writing any row and says so."""
@@ -233,7 +265,7 @@ def test_half_edge_sums_match_fp64(S):
     out = torch.zeros((n, 256), dtype=torch.int16, device="cuda:0")
     assert lib.dmph_nn_edge_eps(_p(pos), _p(rp), _p(ci), _p(eps), n, C.c_float(sur.radius), s) == 0
     assert lib.dmph_nn_edge_fwd_f16(_p(abd), _p(rp), _p(ci), _p(eps), _p(Wcd), _p(bcd), _p(out[:, 128:]), 256, n,
-                                    s) == 0
+                                    None, 0, s) == 0
     got = _bf16_float(out.cpu().numpy().view(np.uint16)[:, 128:])
     assert not out[:, :128].any()   # the left half untouched
     P = pos.cpu().numpy().astype(np.float64)
