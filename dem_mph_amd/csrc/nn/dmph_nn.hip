@@ -236,6 +236,154 @@ static unsigned gemm_grid_x(int M, int rows, bool persistent) {
 }
 
 // ---------------------------------------------------------------------------------------------
+// One round's node update at inference with the intermediates in registers:
+//   u  = relu(W1 [h | S] + b1)     W1 [128][256] (the folded node layer)
+//   h' = h + W2 u + b2             W2 [128][128]; h' stored bf16 into Hn
+//   AB = Wab h'                    Wab [256][128], stored IEEE half (the next round's [A | B])
+// All three weights sit in LDS (160 KB, one workgroup of 16 waves per CU, each
+// wave 16 rows of a 256-row tile, tiles walked persistently).  An MFMA result
+// tile leaves lane (r, g) with 4 adjacent outputs 16c + 4g + [0, 4) of row r, so
+// the 8 values a lane feeds the next product at k step kk are outputs
+// 32kk + 4g + [0, 4) and 32kk + 16 + 4g + [0, 4): W2 and Wab are staged with
+// their columns in that order (pk_col), which makes the product the same sum.
+// ---------------------------------------------------------------------------------------------
+
+constexpr int kNodeWaves = 16;
+constexpr size_t kNodeLds = (size_t)(128 * 256 + 128 * 128 + 256 * 128) * sizeof(bf16_t);   // 160 KB
+
+// source column of staged column k (within its 32-column step) for a product fed by MFMA outputs
+__device__ __forceinline__ int pk_col(int k) {
+    const int base = k & ~31, g = (k >> 3) & 3, t = k & 7;
+    return base + (t < 4 ? 4 * g + t : 16 + 4 * g + t - 4);
+}
+
+__global__ void __launch_bounds__(kNodeWaves * 64) k_node(const bf16_t *__restrict__ HS, int ldhs,
+                                                          const bf16_t *__restrict__ W1, const float *__restrict__ b1,
+                                                          const bf16_t *__restrict__ W2, const float *__restrict__ b2,
+                                                          const bf16_t *__restrict__ Wab, bf16_t *__restrict__ Hn,
+                                                          int ldhn, bf16_t *__restrict__ ABo, int M) {
+    extern __shared__ bf16_t wl[];
+    bf16_t *w1 = wl, *w2 = wl + 128 * 256, *wab = w2 + 128 * 128;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lr = lane & 15, g = lane >> 4, lk = 8 * g;
+    const int ntiles = (M + kNodeWaves * 16 - 1) / (kNodeWaves * 16);
+    int tile = blockIdx.x;
+    typedef uint4 frag_t;
+    frag_t xf[8];
+    auto load_x = [&](int row0) {
+        const int r = row0 + lr;
+#pragma unroll
+        for (int kk = 0; kk < 8; kk++) {
+            xf[kk] = frag_t{0, 0, 0, 0};
+            if (r < M) xf[kk] = *reinterpret_cast<const frag_t *>(HS + (size_t)r * ldhs + kk * 32 + lk);
+        }
+    };
+    load_x((tile * kNodeWaves + wave) * 16);
+    for (int t = threadIdx.x; t < 128 * 256 / 8; t += blockDim.x) {   // W1 as is
+        const int r = t / 32, c = (t % 32) * 8;
+        *reinterpret_cast<uint4 *>(w1 + wl_at(r, c, 256)) = *reinterpret_cast<const uint4 *>(W1 + (size_t)r * 256 + c);
+    }
+    for (int t = threadIdx.x; t < 384 * 128 / 8; t += blockDim.x) {   // W2 and Wab, columns in pk_col order
+        const int r = t / 16, c = (t % 16) * 8;
+        const bf16_t *src = r < 128 ? W2 + (size_t)r * 128 : Wab + (size_t)(r - 128) * 128;
+        bf16_t *dst = r < 128 ? w2 : wab;
+        const int rr = r < 128 ? r : r - 128;
+        const uint2 lo = *reinterpret_cast<const uint2 *>(src + pk_col(c));
+        const uint2 hi = *reinterpret_cast<const uint2 *>(src + pk_col(c + 4));
+        *reinterpret_cast<uint4 *>(dst + wl_at(rr, c, 128)) = make_uint4(lo.x, lo.y, hi.x, hi.y);
+    }
+    __syncthreads();
+    while (true) {
+        const int row0 = (tile * kNodeWaves + wave) * 16, r = row0 + lr;
+        int lb = lr;
+        asm volatile("" : "+v"(lb));   // keep the LDS fragment reads inside the tile loop
+        f4_t acc[8];
+        // u = relu(W1 [h | S] + b1)
+#pragma unroll
+        for (int c = 0; c < 8; c++) acc[c] = f4_t{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kk = 0; kk < 8; kk++)
+#pragma unroll
+            for (int c = 0; c < 8; c++) {
+                const frag_t v = *reinterpret_cast<const frag_t *>(w1 + wl_at(c * 16 + lb, kk * 32 + lk, 256));
+                acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(b8_t, v),
+                                                                 __builtin_bit_cast(b8_t, xf[kk]), acc[c], 0, 0, 0);
+            }
+        uint32_t uf[8][2];   // tile c of u, 4 bf16 = 2 words
+#pragma unroll
+        for (int c = 0; c < 8; c++) {
+            const float4 b = *reinterpret_cast<const float4 *>(b1 + c * 16 + 4 * g);
+            const float v0 = fmaxf(acc[c][0] + b.x, 0.f), v1 = fmaxf(acc[c][1] + b.y, 0.f);
+            const float v2 = fmaxf(acc[c][2] + b.z, 0.f), v3 = fmaxf(acc[c][3] + b.w, 0.f);
+            uf[c][0] = (uint32_t)f2bf(v0) | (uint32_t)f2bf(v1) << 16;
+            uf[c][1] = (uint32_t)f2bf(v2) | (uint32_t)f2bf(v3) << 16;
+        }
+        // the residual h in the output layout, requested before the second product
+        uint2 hr[8];
+#pragma unroll
+        for (int c = 0; c < 8; c++) {
+            hr[c] = make_uint2(0, 0);
+            if (r < M) hr[c] = *reinterpret_cast<const uint2 *>(HS + (size_t)r * ldhs + c * 16 + 4 * g);
+        }
+        // h' = h + W2 u + b2
+#pragma unroll
+        for (int c = 0; c < 8; c++) acc[c] = f4_t{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kk = 0; kk < 4; kk++) {
+            const frag_t uk = make_uint4(uf[2 * kk][0], uf[2 * kk][1], uf[2 * kk + 1][0], uf[2 * kk + 1][1]);
+#pragma unroll
+            for (int c = 0; c < 8; c++) {
+                const frag_t v = *reinterpret_cast<const frag_t *>(w2 + wl_at(c * 16 + lb, kk * 32 + lk, 128));
+                acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(b8_t, v),
+                                                                 __builtin_bit_cast(b8_t, uk), acc[c], 0, 0, 0);
+            }
+        }
+        const int next = tile + (int)gridDim.x;
+        if (next < ntiles) load_x((next * kNodeWaves + wave) * 16);   // the next tile's rows, under what follows
+        uint32_t hf[8][2];
+#pragma unroll
+        for (int c = 0; c < 8; c++) {
+            const float4 b = *reinterpret_cast<const float4 *>(b2 + c * 16 + 4 * g);
+            const float v0 = acc[c][0] + b.x + bf2f((bf16_t)(hr[c].x & 0xffff));
+            const float v1 = acc[c][1] + b.y + bf2f((bf16_t)(hr[c].x >> 16));
+            const float v2 = acc[c][2] + b.z + bf2f((bf16_t)(hr[c].y & 0xffff));
+            const float v3 = acc[c][3] + b.w + bf2f((bf16_t)(hr[c].y >> 16));
+            hf[c][0] = (uint32_t)f2bf(v0) | (uint32_t)f2bf(v1) << 16;
+            hf[c][1] = (uint32_t)f2bf(v2) | (uint32_t)f2bf(v3) << 16;
+            if (r < M) *reinterpret_cast<uint2 *>(Hn + (size_t)r * ldhn + c * 16 + 4 * g) = make_uint2(hf[c][0], hf[c][1]);
+        }
+        // AB = Wab h', in two halves of 128 outputs
+#pragma unroll
+        for (int half = 0; half < 2; half++) {
+#pragma unroll
+            for (int c = 0; c < 8; c++) acc[c] = f4_t{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int kk = 0; kk < 4; kk++) {
+                const frag_t hk = make_uint4(hf[2 * kk][0], hf[2 * kk][1], hf[2 * kk + 1][0], hf[2 * kk + 1][1]);
+#pragma unroll
+                for (int c = 0; c < 8; c++) {
+                    const frag_t v =
+                        *reinterpret_cast<const frag_t *>(wab + wl_at(half * 128 + c * 16 + lb, kk * 32 + lk, 128));
+                    acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(b8_t, v),
+                                                                     __builtin_bit_cast(b8_t, hk), acc[c], 0, 0, 0);
+                }
+            }
+            if (r < M)
+#pragma unroll
+                for (int c = 0; c < 8; c++) {
+                    uint32_t o[4];
+#pragma unroll
+                    for (int e = 0; e < 4; e++) o[e] = (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)acc[c][e]);
+                    *reinterpret_cast<uint2 *>(ABo + (size_t)r * 256 + half * 128 + c * 16 + 4 * g) =
+                        make_uint2(o[0] | o[1] << 16, o[2] | o[3] << 16);
+                }
+        }
+        if (next >= ntiles) break;
+        tile = next;
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
 // dW[N][K] = dY^T X, db[N] = sum_m dY: workgroup g sums rows [g*chunk, (g+1)*chunk) in
 // 64-row tiles staged transposed in LDS (dYt[n][m], Xt[k][m]) and writes FP32
 // partials P[g][N][K] and Pb[g][N]; reduce_rows adds the partials.  The output
@@ -984,6 +1132,24 @@ int dmph_nn_gemm_head(const uint16_t *X, int ldx, const uint16_t *W, const float
     else
         hipLaunchKernelGGL((k_gemm<16, 128, RB, NW, 1, true, true>), grid, dim3(NW * 64), lds, s, X, ldx, W, bias,
                            nullptr, 0, nullptr, 0, M, DMPH_NN_RELU, Wd, bd, Y3);
+    return launch_ok();
+}
+
+int dmph_nn_node(const uint16_t *HS, int ldhs, const uint16_t *W1, const float *b1, const uint16_t *W2, const float *b2,
+                 const uint16_t *Wab, uint16_t *Hn, int ldhn, uint16_t *AB, int M, void *stream) {
+    if (M <= 0) return 0;
+    if (ldhs % 8 || ldhn % 4) return -1;
+    static bool lds_set = false;   // the launch needs all 160 KB of LDS
+    if (!lds_set) {
+        if (hipFuncSetAttribute(reinterpret_cast<const void *>(&k_node), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)kNodeLds) != hipSuccess)
+            return -5;
+        lds_set = true;
+    }
+    const unsigned tiles = (unsigned)((M + kNodeWaves * 16 - 1) / (kNodeWaves * 16));
+    const unsigned grid = tiles < 256u ? tiles : 256u;   // one resident workgroup per CU
+    hipLaunchKernelGGL(k_node, dim3(grid), dim3(kNodeWaves * 64), kNodeLds, (hipStream_t)stream, HS, ldhs, W1, b1, W2,
+                       b2, Wab, Hn, ldhn, AB, M);
     return launch_ok();
 }
 

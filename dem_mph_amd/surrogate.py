@@ -399,10 +399,21 @@ class Surrogate:
             _chk(lib.dmph_nn_edge_eps(_p(pos), _p(rp), _p(ci), _p(ws["eps"]), N, C.c_float(self.radius), s),
                  "edge_eps")
             eps = ws["eps"]
+        fused = not train and self.kstep == 32
         for r, L in enumerate(self.rounds):
             hin = ws["hin"][hslot(r)]
             ab, sr, u = ws["ab"][slot(r)], ws["s"][slot(r)], ws["u"][slot(r)]
-            self._gemm(hin, 2 * W, L["ab"], ab, 2 * W, N, False, half=not train)
+            if not (fused and r > 0):   # else the previous round's node kernel wrote this round's [A | B]
+                self._gemm(hin, 2 * W, L["ab"], ab, 2 * W, N, False, half=not train)
+            if fused and r + 1 < ROUNDS:
+                _chk(lib.dmph_nn_edge_fwd_f16(_p(ab), _p(rp), _p(ci), _p(eps), _p(L["c"].w), _p(L["c"].b),
+                                              _p(hin[:, W:]), 2 * W, N, _p(rows[0] if rows else None),
+                                              rows[1] if rows else 0, s), "edge_fwd_f16")
+                nxt = self.rounds[r + 1]["ab"]
+                _chk(lib.dmph_nn_node(_p(hin), 2 * W, _p(folded[r].wb), _p(folded[r].b), _p(L["n2"].wb),
+                                      _p(L["n2"].b), _p(nxt.wb), _p(ws["hin"][hslot(r + 1)]), 2 * W, _p(ab), N, s),
+                     "node")
+                continue
             if train:
                 _chk(lib.dmph_nn_edge_fwd(_p(ab), _p(pos), _p(rp), _p(ci), _p(L["c"].w), _p(L["c"].b),
                                           _p(sr), W, N, C.c_float(self.radius), s), "edge_fwd")

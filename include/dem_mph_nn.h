@@ -50,6 +50,14 @@ int dmph_nn_gemm(const uint16_t *X, int ldx, const uint16_t *W, const float *bia
 int dmph_nn_gemm_head(const uint16_t *X, int ldx, const uint16_t *W, const float *bias, const float *Wd,
                       const float *bd, float *Y3, int M, int kstep, void *stream);
 
+/* one round's node update at inference (kstep 32), intermediates in registers:
+ * u = relu(W1 HS + b1) (HS [M][256] = [h | S], row stride ldhs; W1 [128][256]),
+ * Hn = h + W2 u + b2 (bf16, [M][128] at row stride ldhn; W2 [128][128]),
+ * AB = Wab Hn (IEEE half [M][256]; Wab [256][128]).  Equal to dmph_nn_gemm three
+ * times up to the order of FP32 sums.  Uses 160 KB of LDS per workgroup. */
+int dmph_nn_node(const uint16_t *HS, int ldhs, const uint16_t *W1, const float *b1, const uint16_t *W2, const float *b2,
+                 const uint16_t *Wab, uint16_t *Hn, int ldhn, uint16_t *AB, int M, void *stream);
+
 /* dW[N][K] (+)= dY^T X, db[N] (+)= sum_m dY (db may be NULL); K in {32, 128, 256},
  * N % 128 == 0; `groups` row chunks summed in a fixed order (deterministic);
  * scratch of dmph_nn_dw_scratch bytes */

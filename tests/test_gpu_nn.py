@@ -102,6 +102,38 @@ def test_gemm_head_matches_gemm_then_head(S, kstep, M):
     np.testing.assert_allclose(c, a, rtol=1e-5, atol=1e-5 * np.abs(a).max())
 
 
+@pytest.mark.parametrize("M", [301, 200_003])
+def test_node_update_matches_three_gemms(S, M):
+    """dmph_nn_node (u, h' and the next [A | B] in registers, W2 and Wab staged
+    with permuted columns) against dmph_nn_gemm run three times on the same
+    bf16 inputs: h' and AB agree to the rounding of their stored values."""
+    rng = np.random.default_rng(M)
+    HS = rng.standard_normal((M, 256)).astype(np.float32)
+    W1 = (rng.standard_normal((128, 256)) / 16).astype(np.float32)
+    W2 = (rng.standard_normal((128, 128)) / 11).astype(np.float32)
+    Wab = (rng.standard_normal((256, 128)) / 11).astype(np.float32)
+    b1, b2 = (torch.tensor(rng.standard_normal(128).astype(np.float32), device="cuda:0") for _ in range(2))
+    HSd, W1d, W2d, Wabd = _dev_bf16(HS), _dev_bf16(W1), _dev_bf16(W2), _dev_bf16(Wab)
+    lib = S.nnlib()
+    s = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    u = torch.zeros((M, 128), dtype=torch.int16, device="cuda:0")
+    h1 = torch.zeros((M, 128), dtype=torch.int16, device="cuda:0")
+    ab1 = torch.zeros((M, 256), dtype=torch.int16, device="cuda:0")
+    assert lib.dmph_nn_gemm(_p(HSd), 256, _p(W1d), _p(b1), None, 0, _p(u), 128, M, 256, 128, S.GEMM_RELU, 32, s) == 0
+    assert lib.dmph_nn_gemm(_p(u), 128, _p(W2d), _p(b2), _p(HSd), 256, _p(h1), 128, M, 128, 128, 0, 32, s) == 0
+    assert lib.dmph_nn_gemm(_p(h1), 128, _p(Wabd), None, None, 0, _p(ab1), 256, M, 128, 256, S.GEMM_F16_OUT, 32, s) == 0
+    h2 = torch.zeros((M, 256), dtype=torch.int16, device="cuda:0")   # h' into the left half of a [h | S] row
+    ab2 = torch.full((M, 256), -1, dtype=torch.int16, device="cuda:0")
+    assert lib.dmph_nn_node(_p(HSd), 256, _p(W1d), _p(b1), _p(W2d), _p(b2), _p(Wabd), _p(h2), 256, _p(ab2), M, s) == 0
+    a, b = _host_bf16(h1), _host_bf16(h2[:, :128])
+    assert not h2[:, 128:].any()
+    assert np.abs(a - b).max() <= 2e-2 * np.abs(a).max() and np.abs(a - b).mean() <= 1e-3 * np.abs(a).mean()
+    x = ab1.cpu().numpy().view(np.float16).astype(np.float64)
+    y = ab2.cpu().numpy().view(np.float16).astype(np.float64)
+    assert np.isfinite(y).all()
+    assert np.abs(x - y).max() <= 2e-2 * np.abs(x).max() and np.abs(x - y).mean() <= 2e-3 * np.abs(x).mean()
+
+
 @pytest.mark.parametrize("K,N", [(32, 128), (128, 128), (256, 128), (128, 256)])
 def test_gemm_dw_matches_numpy(S, K, N):
     rng = np.random.default_rng(K * N)
