@@ -575,6 +575,10 @@ __global__ void __launch_bounds__(256) k_edge_eps(const float *__restrict__ pos,
 // added to the FP32 sums every 16 edges.  The node index, row bounds, column
 // indices and edge features are wave-uniform (scalar loads); B_j is one 4-byte
 // load per lane.
+// BUF (rows of AB within 4 GB: N <= 8M nodes): B_j through one buffer
+// descriptor per wave, the row offset j * 512 as the load's scalar offset, so
+// a gather costs no vector instruction (else a 64-bit address add per edge).
+template <bool BUF>
 __global__ void __launch_bounds__(256) k_edge_fwd_h(const h2_t *__restrict__ AB, const int *__restrict__ rp,
                                                     const int *__restrict__ ci, const uint2 *__restrict__ eps,
                                                     const float *__restrict__ Wc, const float *__restrict__ bc,
@@ -590,11 +594,16 @@ __global__ void __launch_bounds__(256) k_edge_fwd_h(const h2_t *__restrict__ AB,
     const h2_t a2 = {(_Float16)((float)ai.x + bc[f]), (_Float16)((float)ai.y + bc[f + 1])};
     const h2_t zero = {(_Float16)0.f, (_Float16)0.f};
     const uint32_t lane_bytes = 4u * lane;
-    // B_j pair of this lane: uniform row address + lane offset (a buffer-descriptor
-    // load, whose address needs no vector op, measured 17 % slower here)
+    const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<char *>(reinterpret_cast<const char *>(AB) + 256), 0, -1, 0x00020000);
+    // B_j pair of this lane
     auto b_of = [&](int j) {
-        const char *row = reinterpret_cast<const char *>(AB) + (size_t)j * 512 + 256;
-        return *reinterpret_cast<const h2_t *>(row + lane_bytes);
+        if constexpr (BUF) {
+            return bitsh2(__builtin_amdgcn_raw_buffer_load_b32(rsrc, (int)lane_bytes, j * 512, 0));
+        } else {
+            const char *row = reinterpret_cast<const char *>(AB) + (size_t)j * 512 + 256;
+            return *reinterpret_cast<const h2_t *>(row + lane_bytes);
+        }
     };
     float s0 = 0.f, s1 = 0.f;
     int q = deg ? i * row_cap : rp[i];   // rows of a fixed capacity (row_cap entries, deg used) or CSR
@@ -1200,9 +1209,15 @@ int dmph_nn_edge_fwd_f16(const uint16_t *AB, const int *rp, const int *ci, const
                          const float *bc, uint16_t *S, int lds, int N, const int *deg, int row_cap, void *stream) {
     if (N <= 0) return 0;
     if (lds < 128 || lds % 2 || (deg && (row_cap <= 0 || (long long)N * row_cap > 0x7fffffffLL))) return -1;
-    hipLaunchKernelGGL(k_edge_fwd_h, dim3((N + 3) / 4), dim3(256), 0, (hipStream_t)stream,
-                       reinterpret_cast<const h2_t *>(AB), rp, ci, reinterpret_cast<const uint2 *>(eps), Wc, bc, S, lds,
-                       N, deg, row_cap);
+    const bool buf = (size_t)N * 512 <= 0xffffffffull - 1024;   // row offsets fit the 32-bit scalar offset
+    if (buf)
+        hipLaunchKernelGGL(k_edge_fwd_h<true>, dim3((N + 3) / 4), dim3(256), 0, (hipStream_t)stream,
+                           reinterpret_cast<const h2_t *>(AB), rp, ci, reinterpret_cast<const uint2 *>(eps), Wc, bc, S,
+                           lds, N, deg, row_cap);
+    else
+        hipLaunchKernelGGL(k_edge_fwd_h<false>, dim3((N + 3) / 4), dim3(256), 0, (hipStream_t)stream,
+                           reinterpret_cast<const h2_t *>(AB), rp, ci, reinterpret_cast<const uint2 *>(eps), Wc, bc, S,
+                           lds, N, deg, row_cap);
     return launch_ok();
 }
 
