@@ -257,16 +257,25 @@ __device__ __forceinline__ int pk_col(int k) {
     return base + (t < 4 ? 4 * g + t : 16 + 4 * g + t - 4);
 }
 
-__global__ void __launch_bounds__(kNodeWaves * 64) k_node(const bf16_t *__restrict__ HS, int ldhs,
+// DEC (the last round): the third product is the decoder's hidden layer
+// instead, g = relu(Wab h' + bab) with Wab [128][128], and its bf16-rounded
+// values go through the head, Y3 = Wd2 g + bd2 (FP32 [M][3]); h' is not stored.
+template <bool DEC = false, int NWV = DEC ? 8 : kNodeWaves>   // DEC: 8 waves (more registers per wave, no spills)
+__global__ void __launch_bounds__(NWV * 64) k_node(const bf16_t *__restrict__ HS, int ldhs,
                                                           const bf16_t *__restrict__ W1, const float *__restrict__ b1,
                                                           const bf16_t *__restrict__ W2, const float *__restrict__ b2,
                                                           const bf16_t *__restrict__ Wab, bf16_t *__restrict__ Hn,
-                                                          int ldhn, bf16_t *__restrict__ ABo, int M) {
+                                                          int ldhn, bf16_t *__restrict__ ABo, int M,
+                                                          const float *__restrict__ bab = nullptr,
+                                                          const float *__restrict__ Wd2 = nullptr,
+                                                          const float *__restrict__ bd2 = nullptr,
+                                                          float *__restrict__ Y3 = nullptr) {
+    constexpr int NAB = DEC ? 128 : 256;   // rows of the third weight
     extern __shared__ bf16_t wl[];
     bf16_t *w1 = wl, *w2 = wl + 128 * 256, *wab = w2 + 128 * 128;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int lr = lane & 15, g = lane >> 4, lk = 8 * g;
-    const int ntiles = (M + kNodeWaves * 16 - 1) / (kNodeWaves * 16);
+    const int ntiles = (M + NWV * 16 - 1) / (NWV * 16);
     int tile = blockIdx.x;
     typedef uint4 frag_t;
     frag_t xf[8];
@@ -278,12 +287,12 @@ __global__ void __launch_bounds__(kNodeWaves * 64) k_node(const bf16_t *__restri
             if (r < M) xf[kk] = *reinterpret_cast<const frag_t *>(HS + (size_t)r * ldhs + kk * 32 + lk);
         }
     };
-    load_x((tile * kNodeWaves + wave) * 16);
+    load_x((tile * NWV + wave) * 16);
     for (int t = threadIdx.x; t < 128 * 256 / 8; t += blockDim.x) {   // W1 as is
         const int r = t / 32, c = (t % 32) * 8;
         *reinterpret_cast<uint4 *>(w1 + wl_at(r, c, 256)) = *reinterpret_cast<const uint4 *>(W1 + (size_t)r * 256 + c);
     }
-    for (int t = threadIdx.x; t < 384 * 128 / 8; t += blockDim.x) {   // W2 and Wab, columns in pk_col order
+    for (int t = threadIdx.x; t < (128 + NAB) * 128 / 8; t += blockDim.x) {   // W2 and Wab, columns in pk_col order
         const int r = t / 16, c = (t % 16) * 8;
         const bf16_t *src = r < 128 ? W2 + (size_t)r * 128 : Wab + (size_t)(r - 128) * 128;
         bf16_t *dst = r < 128 ? w2 : wab;
@@ -294,7 +303,7 @@ __global__ void __launch_bounds__(kNodeWaves * 64) k_node(const bf16_t *__restri
     }
     __syncthreads();
     while (true) {
-        const int row0 = (tile * kNodeWaves + wave) * 16, r = row0 + lr;
+        const int row0 = (tile * NWV + wave) * 16, r = row0 + lr;
         int lb = lr;
         asm volatile("" : "+v"(lb));   // keep the LDS fragment reads inside the tile loop
         f4_t acc[8];
@@ -339,7 +348,7 @@ __global__ void __launch_bounds__(kNodeWaves * 64) k_node(const bf16_t *__restri
             }
         }
         const int next = tile + (int)gridDim.x;
-        if (next < ntiles) load_x((next * kNodeWaves + wave) * 16);   // the next tile's rows, under what follows
+        if (!DEC && next < ntiles) load_x((next * NWV + wave) * 16);   // the next tile's rows, under what follows
         uint32_t hf[8][2];
 #pragma unroll
         for (int c = 0; c < 8; c++) {
@@ -350,11 +359,49 @@ __global__ void __launch_bounds__(kNodeWaves * 64) k_node(const bf16_t *__restri
             const float v3 = acc[c][3] + b.w + bf2f((bf16_t)(hr[c].y >> 16));
             hf[c][0] = (uint32_t)f2bf(v0) | (uint32_t)f2bf(v1) << 16;
             hf[c][1] = (uint32_t)f2bf(v2) | (uint32_t)f2bf(v3) << 16;
-            if (r < M) *reinterpret_cast<uint2 *>(Hn + (size_t)r * ldhn + c * 16 + 4 * g) = make_uint2(hf[c][0], hf[c][1]);
+            if (!DEC && r < M)
+                *reinterpret_cast<uint2 *>(Hn + (size_t)r * ldhn + c * 16 + 4 * g) = make_uint2(hf[c][0], hf[c][1]);
+        }
+        if constexpr (DEC) {   // g = relu(Wab h' + bab), then the 3-output head
+#pragma unroll
+            for (int c = 0; c < 8; c++) acc[c] = f4_t{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int kk = 0; kk < 4; kk++) {
+                const frag_t hk = make_uint4(hf[2 * kk][0], hf[2 * kk][1], hf[2 * kk + 1][0], hf[2 * kk + 1][1]);
+#pragma unroll
+                for (int c = 0; c < 8; c++) {
+                    const frag_t v = *reinterpret_cast<const frag_t *>(wab + wl_at(c * 16 + lb, kk * 32 + lk, 128));
+                    acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(b8_t, v),
+                                                                     __builtin_bit_cast(b8_t, hk), acc[c], 0, 0, 0);
+                }
+            }
+            float part[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+            for (int c = 0; c < 8; c++) {
+                // (lb - lr) = 0, opaque: the head's weights are re-read per tile, not held in registers
+                const int col = c * 16 + 4 * g + (lb - lr);
+                const float4 b = *reinterpret_cast<const float4 *>(bab + col);
+                const float bb[4] = {b.x, b.y, b.z, b.w};
+#pragma unroll
+                for (int o = 0; o < 3; o++) {
+                    const float4 w = *reinterpret_cast<const float4 *>(Wd2 + o * 128 + col);
+                    const float ww[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+                    for (int e = 0; e < 4; e++) part[o] += bf2f(f2bf(fmaxf(acc[c][e] + bb[e], 0.f))) * ww[e];
+                }
+            }
+#pragma unroll
+            for (int o = 0; o < 3; o++) {
+                part[o] += __shfl_xor(part[o], 16);
+                part[o] += __shfl_xor(part[o], 32);
+            }
+            if (lane < 16 && r < M)
+#pragma unroll
+                for (int o = 0; o < 3; o++) Y3[3 * (size_t)r + o] = part[o] + bd2[o];
         }
         // AB = Wab h', in two halves of 128 outputs
 #pragma unroll
-        for (int half = 0; half < 2; half++) {
+        for (int half = 0; half < (DEC ? 0 : 2); half++) {
 #pragma unroll
             for (int c = 0; c < 8; c++) acc[c] = f4_t{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
@@ -380,6 +427,7 @@ __global__ void __launch_bounds__(kNodeWaves * 64) k_node(const bf16_t *__restri
         }
         if (next >= ntiles) break;
         tile = next;
+        if (DEC) load_x((tile * NWV + wave) * 16);   // (registers: no prefetch under the decoder)
     }
 }
 
@@ -1150,15 +1198,35 @@ int dmph_nn_node(const uint16_t *HS, int ldhs, const uint16_t *W1, const float *
     if (ldhs % 8 || ldhn % 4) return -1;
     static bool lds_set = false;   // the launch needs all 160 KB of LDS
     if (!lds_set) {
-        if (hipFuncSetAttribute(reinterpret_cast<const void *>(&k_node), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)kNodeLds) != hipSuccess)
+        if (hipFuncSetAttribute(reinterpret_cast<const void *>(&k_node<false>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)kNodeLds) != hipSuccess)
             return -5;
         lds_set = true;
     }
     const unsigned tiles = (unsigned)((M + kNodeWaves * 16 - 1) / (kNodeWaves * 16));
     const unsigned grid = tiles < 256u ? tiles : 256u;   // one resident workgroup per CU
-    hipLaunchKernelGGL(k_node, dim3(grid), dim3(kNodeWaves * 64), kNodeLds, (hipStream_t)stream, HS, ldhs, W1, b1, W2,
-                       b2, Wab, Hn, ldhn, AB, M);
+    hipLaunchKernelGGL(k_node<false>, dim3(grid), dim3(kNodeWaves * 64), kNodeLds, (hipStream_t)stream, HS, ldhs, W1,
+                       b1, W2, b2, Wab, Hn, ldhn, AB, M, nullptr, nullptr, nullptr, nullptr);
+    return launch_ok();
+}
+
+int dmph_nn_node_head(const uint16_t *HS, int ldhs, const uint16_t *W1, const float *b1, const uint16_t *W2,
+                      const float *b2, const uint16_t *Wd1, const float *bd1, const float *Wd2, const float *bd2,
+                      float *Y3, int M, void *stream) {
+    if (M <= 0) return 0;
+    if (ldhs % 8) return -1;
+    constexpr size_t lds = (size_t)(128 * 256 + 2 * 128 * 128) * sizeof(bf16_t);   // 128 KB
+    static bool lds_set = false;
+    if (!lds_set) {
+        if (hipFuncSetAttribute(reinterpret_cast<const void *>(&k_node<true>), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)lds) != hipSuccess)
+            return -5;
+        lds_set = true;
+    }
+    const unsigned tiles = (unsigned)((M + 8 * 16 - 1) / (8 * 16));
+    const unsigned grid = tiles < 256u ? tiles : 256u;
+    hipLaunchKernelGGL(k_node<true>, dim3(grid), dim3(8 * 64), lds, (hipStream_t)stream, HS, ldhs, W1, b1, W2,
+                       b2, Wd1, nullptr, 0, nullptr, M, bd1, Wd2, bd2, Y3);
     return launch_ok();
 }
 

@@ -414,6 +414,14 @@ class Surrogate:
                                       _p(L["n2"].b), _p(nxt.wb), _p(ws["hin"][hslot(r + 1)]), 2 * W, _p(ab), N, s),
                      "node")
                 continue
+            if fused:   # the last round and the decoder in one kernel
+                _chk(lib.dmph_nn_edge_fwd_f16(_p(ab), _p(rp), _p(ci), _p(eps), _p(L["c"].w), _p(L["c"].b),
+                                              _p(hin[:, W:]), 2 * W, N, _p(rows[0] if rows else None),
+                                              rows[1] if rows else 0, s), "edge_fwd_f16")
+                _chk(lib.dmph_nn_node_head(_p(hin), 2 * W, _p(folded[r].wb), _p(folded[r].b), _p(L["n2"].wb),
+                                           _p(L["n2"].b), _p(self.dec1.wb), _p(self.dec1.b), _p(self.dec2.w),
+                                           _p(self.dec2.b), _p(ws["y"]), N, s), "node_head")
+                return ws["y"][:N]
             if train:
                 _chk(lib.dmph_nn_edge_fwd(_p(ab), _p(pos), _p(rp), _p(ci), _p(L["c"].w), _p(L["c"].b),
                                           _p(sr), W, N, C.c_float(self.radius), s), "edge_fwd")

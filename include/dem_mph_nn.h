@@ -58,6 +58,14 @@ int dmph_nn_gemm_head(const uint16_t *X, int ldx, const uint16_t *W, const float
 int dmph_nn_node(const uint16_t *HS, int ldhs, const uint16_t *W1, const float *b1, const uint16_t *W2, const float *b2,
                  const uint16_t *Wab, uint16_t *Hn, int ldhn, uint16_t *AB, int M, void *stream);
 
+/* the last round's node update and the decoder in one kernel at inference:
+ * h' as dmph_nn_node (not stored), then Y3 [M][3] = Wd2 relu(Wd1 h' + bd1) + bd2
+ * (Wd1 [128][128] bf16, Wd2 [3][128] FP32; the hidden layer rounded to bf16 as
+ * dmph_nn_gemm_head rounds it).  128 KB of LDS per workgroup. */
+int dmph_nn_node_head(const uint16_t *HS, int ldhs, const uint16_t *W1, const float *b1, const uint16_t *W2,
+                      const float *b2, const uint16_t *Wd1, const float *bd1, const float *Wd2, const float *bd2,
+                      float *Y3, int M, void *stream);
+
 /* dW[N][K] (+)= dY^T X, db[N] (+)= sum_m dY (db may be NULL); K in {32, 128, 256},
  * N % 128 == 0; `groups` row chunks summed in a fixed order (deterministic);
  * scratch of dmph_nn_dw_scratch bytes */

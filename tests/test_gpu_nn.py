@@ -134,6 +134,36 @@ def test_node_update_matches_three_gemms(S, M):
     assert np.abs(x - y).max() <= 2e-2 * np.abs(x).max() and np.abs(x - y).mean() <= 2e-3 * np.abs(x).mean()
 
 
+@pytest.mark.parametrize("M", [301, 200_003])
+def test_node_head_matches_gemms_and_head(S, M):
+    """dmph_nn_node_head (last round + decoder) against dmph_nn_gemm twice and
+    dmph_nn_gemm_head on the same bf16 inputs."""
+    rng = np.random.default_rng(M + 1)
+    HS = rng.standard_normal((M, 256)).astype(np.float32)
+    W1 = (rng.standard_normal((128, 256)) / 16).astype(np.float32)
+    W2 = (rng.standard_normal((128, 128)) / 11).astype(np.float32)
+    Wd1 = (rng.standard_normal((128, 128)) / 11).astype(np.float32)
+    HSd, W1d, W2d, Wd1d = _dev_bf16(HS), _dev_bf16(W1), _dev_bf16(W2), _dev_bf16(Wd1)
+    b1, b2, bd1 = (torch.tensor(rng.standard_normal(128).astype(np.float32), device="cuda:0") for _ in range(3))
+    Wd2 = torch.tensor(rng.standard_normal((3, 128)).astype(np.float32), device="cuda:0")
+    bd2 = torch.tensor(rng.standard_normal(3).astype(np.float32), device="cuda:0")
+    lib = S.nnlib()
+    s = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    u = torch.zeros((M, 128), dtype=torch.int16, device="cuda:0")
+    h = torch.zeros((M, 128), dtype=torch.int16, device="cuda:0")
+    y1 = torch.zeros((M, 3), dtype=torch.float32, device="cuda:0")
+    y2 = torch.full((M, 3), float("nan"), dtype=torch.float32, device="cuda:0")
+    assert lib.dmph_nn_gemm(_p(HSd), 256, _p(W1d), _p(b1), None, 0, _p(u), 128, M, 256, 128, S.GEMM_RELU, 32, s) == 0
+    assert lib.dmph_nn_gemm(_p(u), 128, _p(W2d), _p(b2), _p(HSd), 256, _p(h), 128, M, 128, 128, 0, 32, s) == 0
+    assert lib.dmph_nn_gemm_head(_p(h), 128, _p(Wd1d), _p(bd1), _p(Wd2), _p(bd2), _p(y1), M, 32, s) == 0
+    assert lib.dmph_nn_node_head(_p(HSd), 256, _p(W1d), _p(b1), _p(W2d), _p(b2), _p(Wd1d), _p(bd1), _p(Wd2), _p(bd2),
+                                 _p(y2), M, s) == 0
+    a, b = y1.cpu().numpy(), y2.cpu().numpy()
+    assert np.isfinite(b).all()
+    err = np.abs(a - b)
+    assert err.max() <= 2e-2 * np.abs(a).max() and err.mean() <= 2e-3 * np.abs(a).mean()
+
+
 @pytest.mark.parametrize("K,N", [(32, 128), (128, 128), (256, 128), (128, 256)])
 def test_gemm_dw_matches_numpy(S, K, N):
     rng = np.random.default_rng(K * N)
