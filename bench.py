@@ -303,12 +303,14 @@ def surrogate_bench(D, scenes, seq_ms_per_dt: float, batched_ms_per_dt: float, c
     dt = params["Dt"]
     t_step = 0.0
     finite = True
-    for c0 in range(0, len(scenes), chunk):
-        r = SG.advance_batch(sur, scenes[c0:c0 + chunk], steps, dt)
+    nch = (len(scenes) + chunk - 1) // chunk   # equal chunks of at most `chunk` scenes
+    cuts = np.linspace(0, len(scenes), nch + 1).round().astype(int)
+    for k in range(nch):
+        r = SG.advance_batch(sur, scenes[cuts[k]:cuts[k + 1]], steps, dt)
         t_step += r["seconds_per_step"]
         finite = finite and r["finite"]
-        print("bench: surrogate chunk %d: %.1f ms/step for %d nodes" % (c0 // chunk, r["seconds_per_step"] * 1e3,
-                                                                        r["nodes"]), file=sys.stderr, flush=True)
+        print("bench: surrogate chunk %d: %.1f ms/step for %d nodes" % (k, r["seconds_per_step"] * 1e3, r["nodes"]),
+              file=sys.stderr, flush=True)
     n = sum(sc.n for sc in scenes)
     reanchor = 200
     t_total = t_step + (SG.STRIDE * batched_ms_per_dt * 1e-3) / reanchor

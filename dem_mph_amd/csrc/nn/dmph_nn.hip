@@ -1192,6 +1192,145 @@ int dmph_nn_gemm_head(const uint16_t *X, int ldx, const uint16_t *W, const float
     return launch_ok();
 }
 
+// The encoder at inference, from the raw node data to the first round's input:
+// the 32 node features of k_features (computed per lane: lane (r, g) makes
+// features 8g .. 8g + 7 of row r, its MFMA fragment), h1 = relu(We1 x + be1),
+// h0 = We2 h1 + be2 (stored bf16 into H0, the left half of [h | S]) and
+// AB = Wab h0 (IEEE half), intermediates in registers (as k_node).
+__global__ void __launch_bounds__(kNodeWaves * 64) k_enc(const float *__restrict__ vel_hist, const int *__restrict__ cls,
+                                                         const float *__restrict__ pos, NnFeat ft,
+                                                         const bf16_t *__restrict__ We1, const float *__restrict__ be1,
+                                                         const bf16_t *__restrict__ We2, const float *__restrict__ be2,
+                                                         const bf16_t *__restrict__ Wab, bf16_t *__restrict__ H0,
+                                                         int ldh0, bf16_t *__restrict__ ABo, int M) {
+    extern __shared__ bf16_t wl[];
+    bf16_t *we1 = wl, *we2 = wl + 128 * 32, *wab = we2 + 128 * 128;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lr = lane & 15, g = lane >> 4, lk = 8 * g;
+    const int ntiles = (M + kNodeWaves * 16 - 1) / (kNodeWaves * 16);
+    typedef uint4 frag_t;
+    for (int t = threadIdx.x; t < 128 * 32 / 8; t += blockDim.x) {   // We1 as is
+        const int r = t / 4, c = (t % 4) * 8;
+        *reinterpret_cast<uint4 *>(we1 + wl_at(r, c, 32)) = *reinterpret_cast<const uint4 *>(We1 + (size_t)r * 32 + c);
+    }
+    for (int t = threadIdx.x; t < 384 * 128 / 8; t += blockDim.x) {   // We2 and Wab, columns in pk_col order
+        const int r = t / 16, c = (t % 16) * 8;
+        const bf16_t *src = r < 128 ? We2 + (size_t)r * 128 : Wab + (size_t)(r - 128) * 128;
+        bf16_t *dst = r < 128 ? we2 : wab;
+        const int rr = r < 128 ? r : r - 128;
+        const uint2 lo = *reinterpret_cast<const uint2 *>(src + pk_col(c));
+        const uint2 hi = *reinterpret_cast<const uint2 *>(src + pk_col(c + 4));
+        *reinterpret_cast<uint4 *>(dst + wl_at(rr, c, 128)) = make_uint4(lo.x, lo.y, hi.x, hi.y);
+    }
+    __syncthreads();
+    for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const int r = (tile * kNodeWaves + wave) * 16 + lr;
+        int lb = lr;
+        asm volatile("" : "+v"(lb));
+        // features 8g .. 8g + 7 of row r, as k_features computes them
+        uint32_t xw[4] = {0, 0, 0, 0};
+        if (r < M) {
+            bf16_t x[8];
+#pragma unroll
+            for (int t = 0; t < 8; t++) {
+                const int f = lk + t;
+                float v = 0.f;
+                if (f < 15) v = vel_hist[((size_t)(f / 3) * M + r) * 3 + f % 3] * ft.vscale;
+                else if (f < 18) v = cls[r] == f - 15 ? 1.f : 0.f;
+                else if (f < 24) {
+                    const int k = (f - 18) >> 1;   // selects, not indexing: the parameter struct stays in registers
+                    const float tlo = k == 0 ? ft.tlo[0] : (k == 1 ? ft.tlo[1] : ft.tlo[2]);
+                    const float thi = k == 0 ? ft.thi[0] : (k == 1 ? ft.thi[1] : ft.thi[2]);
+                    const float d = (f & 1) ? (thi - pos[3 * r + k]) * ft.inv_r : (pos[3 * r + k] - tlo) * ft.inv_r;
+                    v = fminf(fmaxf(d, 0.f), 1.f);
+                }
+                x[t] = f < 24 ? f2bf(v) : (bf16_t)0;
+            }
+#pragma unroll
+            for (int t = 0; t < 4; t++) xw[t] = (uint32_t)x[2 * t] | (uint32_t)x[2 * t + 1] << 16;
+        }
+        const frag_t xf = make_uint4(xw[0], xw[1], xw[2], xw[3]);
+        f4_t acc[8];
+#pragma unroll
+        for (int c = 0; c < 8; c++) {
+            const frag_t v = *reinterpret_cast<const frag_t *>(we1 + wl_at(c * 16 + lb, lk, 32));
+            acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(b8_t, v), __builtin_bit_cast(b8_t, xf),
+                                                             f4_t{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+        }
+        uint32_t hf[8][2];
+#pragma unroll
+        for (int c = 0; c < 8; c++) {   // h1 = relu(We1 x + be1)
+            const float4 b = *reinterpret_cast<const float4 *>(be1 + c * 16 + 4 * g);
+            hf[c][0] = (uint32_t)f2bf(fmaxf(acc[c][0] + b.x, 0.f)) | (uint32_t)f2bf(fmaxf(acc[c][1] + b.y, 0.f)) << 16;
+            hf[c][1] = (uint32_t)f2bf(fmaxf(acc[c][2] + b.z, 0.f)) | (uint32_t)f2bf(fmaxf(acc[c][3] + b.w, 0.f)) << 16;
+        }
+#pragma unroll
+        for (int c = 0; c < 8; c++) acc[c] = f4_t{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kk = 0; kk < 4; kk++) {   // h0 = We2 h1 + be2
+            const frag_t hk = make_uint4(hf[2 * kk][0], hf[2 * kk][1], hf[2 * kk + 1][0], hf[2 * kk + 1][1]);
+#pragma unroll
+            for (int c = 0; c < 8; c++) {
+                const frag_t v = *reinterpret_cast<const frag_t *>(we2 + wl_at(c * 16 + lb, kk * 32 + lk, 128));
+                acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(b8_t, v), __builtin_bit_cast(b8_t, hk),
+                                                                 acc[c], 0, 0, 0);
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < 8; c++) {
+            const float4 b = *reinterpret_cast<const float4 *>(be2 + c * 16 + 4 * g);
+            hf[c][0] = (uint32_t)f2bf(acc[c][0] + b.x) | (uint32_t)f2bf(acc[c][1] + b.y) << 16;
+            hf[c][1] = (uint32_t)f2bf(acc[c][2] + b.z) | (uint32_t)f2bf(acc[c][3] + b.w) << 16;
+            if (r < M) *reinterpret_cast<uint2 *>(H0 + (size_t)r * ldh0 + c * 16 + 4 * g) = make_uint2(hf[c][0], hf[c][1]);
+        }
+#pragma unroll
+        for (int half = 0; half < 2; half++) {   // AB = Wab h0
+#pragma unroll
+            for (int c = 0; c < 8; c++) acc[c] = f4_t{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int kk = 0; kk < 4; kk++) {
+                const frag_t hk = make_uint4(hf[2 * kk][0], hf[2 * kk][1], hf[2 * kk + 1][0], hf[2 * kk + 1][1]);
+#pragma unroll
+                for (int c = 0; c < 8; c++) {
+                    const frag_t v =
+                        *reinterpret_cast<const frag_t *>(wab + wl_at(half * 128 + c * 16 + lb, kk * 32 + lk, 128));
+                    acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(b8_t, v),
+                                                                     __builtin_bit_cast(b8_t, hk), acc[c], 0, 0, 0);
+                }
+            }
+            if (r < M)
+#pragma unroll
+                for (int c = 0; c < 8; c++) {
+                    uint32_t o[4];
+#pragma unroll
+                    for (int e = 0; e < 4; e++) o[e] = (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)acc[c][e]);
+                    *reinterpret_cast<uint2 *>(ABo + (size_t)r * 256 + half * 128 + c * 16 + 4 * g) =
+                        make_uint2(o[0] | o[1] << 16, o[2] | o[3] << 16);
+                }
+        }
+    }
+}
+
+int dmph_nn_encode(const float *vel_hist, int N, const int *cls, const float *pos, const NnFeat *ft, const uint16_t *We1,
+                   const float *be1, const uint16_t *We2, const float *be2, const uint16_t *Wab, uint16_t *H0, int ldh0,
+                   uint16_t *AB, void *stream) {
+    if (N <= 0) return 0;
+    if (ldh0 % 4) return -1;
+    constexpr size_t lds = (size_t)(128 * 32 + 128 * 128 + 256 * 128) * sizeof(bf16_t);   // 104 KB
+    static bool lds_set = false;
+    if (!lds_set) {
+        if (hipFuncSetAttribute(reinterpret_cast<const void *>(&k_enc), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)lds) != hipSuccess)
+            return -5;
+        lds_set = true;
+    }
+    const unsigned tiles = (unsigned)((N + kNodeWaves * 16 - 1) / (kNodeWaves * 16));
+    const unsigned grid = tiles < 256u ? tiles : 256u;
+    hipLaunchKernelGGL(k_enc, dim3(grid), dim3(kNodeWaves * 64), lds, (hipStream_t)stream, vel_hist, cls, pos, *ft, We1,
+                       be1, We2, be2, Wab, H0, ldh0, AB, N);
+    return launch_ok();
+}
+
 int dmph_nn_node(const uint16_t *HS, int ldhs, const uint16_t *W1, const float *b1, const uint16_t *W2, const float *b2,
                  const uint16_t *Wab, uint16_t *Hn, int ldhn, uint16_t *AB, int M, void *stream) {
     if (M <= 0) return 0;

@@ -329,15 +329,18 @@ class Surrogate:
         """x0 [N, 32] bf16 from vel_hist [5, N, 3] (newest first), class ids, positions."""
         N = pos.shape[0]
         ws = self._workspace(N, train)
+        _chk(nnlib().dmph_nn_features(_p(vel_hist), N, _p(cls), _p(pos), C.byref(self._feat()), _p(ws["x0"]),
+                                      self._s()), "features")
+        return ws["x0"]
+
+    def _feat(self) -> "NnFeat":
         ft = NnFeat()
         ft.vscale = self.vscale
         for k in range(3):
             ft.tlo[k] = float(self.tank_lo[k])
             ft.thi[k] = float(self.tank_hi[k])
         ft.inv_r = 1.0 / self.radius
-        _chk(nnlib().dmph_nn_features(_p(vel_hist), N, _p(cls), _p(pos), C.byref(ft), _p(ws["x0"]), self._s()),
-             "features")
-        return ws["x0"]
+        return ft
 
     def _s(self) -> C.c_void_p:
         return C.c_void_p(self.torch.cuda.current_stream(self.dev).cuda_stream)
@@ -380,7 +383,7 @@ class Surrogate:
             self._folded = out
         return self._folded
 
-    def forward(self, x0, pos, rp, ci, N: int, train: bool = True, eps=None, rows=None):
+    def forward(self, x0, pos, rp, ci, N: int, train: bool = True, eps=None, rows=None, feats=None):
         """Accelerations (scaled) y [N, 3] FP32; with `train` every round's
         activations are kept for backward, else two slots alternate and the
         aggregation's linear layer is folded into the node MLP (_fold)."""
@@ -390,8 +393,15 @@ class Surrogate:
         s = self._s()
         slot = (lambda r: r) if train else (lambda r: 0)
         hslot = (lambda r: r) if train else (lambda r: r % 2)
-        self._gemm(x0, FEAT, self.enc1, ws["h1"], W, N, True)
-        self._gemm(ws["h1"], W, self.enc2, ws["hin"][0], 2 * W, N, False)
+        fused = not train and self.kstep == 32
+        if fused and feats is not None:   # features, encoder and the first [A | B] in one kernel
+            vel_hist, cls = feats
+            _chk(lib.dmph_nn_encode(_p(vel_hist), N, _p(cls), _p(pos), C.byref(self._feat()), _p(self.enc1.wb),
+                                    _p(self.enc1.b), _p(self.enc2.wb), _p(self.enc2.b), _p(self.rounds[0]["ab"].wb),
+                                    _p(ws["hin"][0]), 2 * W, _p(ws["ab"][0]), s), "encode")
+        else:
+            self._gemm(x0, FEAT, self.enc1, ws["h1"], W, N, True)
+            self._gemm(ws["h1"], W, self.enc2, ws["hin"][0], 2 * W, N, False)
         folded = None if train else self._fold()
         if not train and eps is None:   # edge features once per graph, for the half-precision edge sums
             if ws.get("eps") is None or ws["eps"].numel() < 2 * ci.numel():
@@ -399,11 +409,10 @@ class Surrogate:
             _chk(lib.dmph_nn_edge_eps(_p(pos), _p(rp), _p(ci), _p(ws["eps"]), N, C.c_float(self.radius), s),
                  "edge_eps")
             eps = ws["eps"]
-        fused = not train and self.kstep == 32
         for r, L in enumerate(self.rounds):
             hin = ws["hin"][hslot(r)]
             ab, sr, u = ws["ab"][slot(r)], ws["s"][slot(r)], ws["u"][slot(r)]
-            if not (fused and r > 0):   # else the previous round's node kernel wrote this round's [A | B]
+            if not (fused and (r > 0 or feats is not None)):   # else a fused kernel wrote this round's [A | B]
                 self._gemm(hin, 2 * W, L["ab"], ab, 2 * W, N, False, half=not train)
             if fused and r + 1 < ROUNDS:
                 _chk(lib.dmph_nn_edge_fwd_f16(_p(ab), _p(rp), _p(ci), _p(eps), _p(L["c"].w), _p(L["c"].b),
@@ -506,12 +515,14 @@ class Surrogate:
         int32 offsets, else CSR), features, network; returns y [N, 3]."""
         N = pos.shape[0]
         r = self.graph_rows(pos, scene, nscene)
-        x0 = self.features(vel_hist, cls, pos, train=False)
+        fused = self.kstep == 32
+        x0 = None if fused else self.features(vel_hist, cls, pos, train=False)
+        feats = (vel_hist, cls) if fused else None
         if r is not None:
             ci, deg, cap, eps = r
-            return self.forward(x0, pos, None, ci, N, train=False, eps=eps, rows=(deg, cap))
+            return self.forward(x0, pos, None, ci, N, train=False, eps=eps, rows=(deg, cap), feats=feats)
         rp, ci, _ = self.graph(pos, scene, nscene, eps=True)
-        return self.forward(x0, pos, rp, ci, N, train=False, eps=self.graph_eps)
+        return self.forward(x0, pos, rp, ci, N, train=False, eps=self.graph_eps, feats=feats)
 
     def rollout_step(self, pos, vel_hist, y, cls, body, nbodies: int, h: float, body_sums) -> None:
         """x, v history advanced by stride h from the network's output y."""

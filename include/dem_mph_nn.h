@@ -58,6 +58,13 @@ int dmph_nn_gemm_head(const uint16_t *X, int ldx, const uint16_t *W, const float
 int dmph_nn_node(const uint16_t *HS, int ldhs, const uint16_t *W1, const float *b1, const uint16_t *W2, const float *b2,
                  const uint16_t *Wab, uint16_t *Hn, int ldhn, uint16_t *AB, int M, void *stream);
 
+/* the encoder at inference in one kernel: the features of dmph_nn_features,
+ * then H0 = We2 relu(We1 x + be1) + be2 (bf16, row stride ldh0; We1 [128][32],
+ * We2 [128][128]) and AB = Wab H0 (IEEE half [N][256]; Wab [256][128]) */
+int dmph_nn_encode(const float *vel_hist, int N, const int *cls, const float *pos, const NnFeat *ft, const uint16_t *We1,
+                   const float *be1, const uint16_t *We2, const float *be2, const uint16_t *Wab, uint16_t *H0, int ldh0,
+                   uint16_t *AB, void *stream);
+
 /* the last round's node update and the decoder in one kernel at inference:
  * h' as dmph_nn_node (not stored), then Y3 [M][3] = Wd2 relu(Wd1 h' + bd1) + bd2
  * (Wd1 [128][128] bf16, Wd2 [3][128] FP32; the hidden layer rounded to bf16 as

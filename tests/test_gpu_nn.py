@@ -164,6 +164,34 @@ def test_node_head_matches_gemms_and_head(S, M):
     assert err.max() <= 2e-2 * np.abs(a).max() and err.mean() <= 2e-3 * np.abs(a).mean()
 
 
+def test_encode_matches_features_and_gemms(S):
+    """dmph_nn_encode (features, both encoder layers and the first [A | B] in
+    one kernel) against dmph_nn_features and dmph_nn_gemm three times, on a
+    scene of 3 classes with wall distances inside and outside the clip."""
+    sur, sc, pos, vh, cls, tgt, mask = _small_problem(S, seed=5)
+    n = sc.n
+    lib = S.nnlib()
+    s = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    x0 = sur.features(vh, cls, pos, train=False)
+    h1 = torch.zeros((n, 128), dtype=torch.int16, device="cuda:0")
+    h0 = torch.zeros((n, 256), dtype=torch.int16, device="cuda:0")
+    ab1 = torch.zeros((n, 256), dtype=torch.int16, device="cuda:0")
+    L0 = sur.rounds[0]["ab"]
+    assert lib.dmph_nn_gemm(_p(x0), 32, _p(sur.enc1.wb), _p(sur.enc1.b), None, 0, _p(h1), 128, n, 32, 128, S.GEMM_RELU,
+                            32, s) == 0
+    assert lib.dmph_nn_gemm(_p(h1), 128, _p(sur.enc2.wb), _p(sur.enc2.b), None, 0, _p(h0), 256, n, 128, 128, 0, 32, s) == 0
+    assert lib.dmph_nn_gemm(_p(h0), 256, _p(L0.wb), None, None, 0, _p(ab1), 256, n, 128, 256, S.GEMM_F16_OUT, 32, s) == 0
+    h0b = torch.zeros((n, 256), dtype=torch.int16, device="cuda:0")
+    ab2 = torch.zeros((n, 256), dtype=torch.int16, device="cuda:0")
+    assert lib.dmph_nn_encode(_p(vh), n, _p(cls), _p(pos), C.byref(sur._feat()), _p(sur.enc1.wb), _p(sur.enc1.b),
+                              _p(sur.enc2.wb), _p(sur.enc2.b), _p(L0.wb), _p(h0b), 256, _p(ab2), s) == 0
+    a, b = _host_bf16(h0[:, :128]), _host_bf16(h0b[:, :128])
+    assert np.abs(a - b).max() <= 2e-2 * np.abs(a).max() and np.abs(a - b).mean() <= 1e-3 * np.abs(a).mean()
+    x = ab1.cpu().numpy().view(np.float16).astype(np.float64)
+    y = ab2.cpu().numpy().view(np.float16).astype(np.float64)
+    assert np.abs(x - y).max() <= 2e-2 * np.abs(x).max() and np.abs(x - y).mean() <= 2e-3 * np.abs(x).mean()
+
+
 @pytest.mark.parametrize("K,N", [(32, 128), (128, 128), (256, 128), (128, 256)])
 def test_gemm_dw_matches_numpy(S, K, N):
     rng = np.random.default_rng(K * N)
