@@ -260,7 +260,9 @@ __device__ __forceinline__ int pk_col(int k) {
 // DEC (the last round): the third product is the decoder's hidden layer
 // instead, g = relu(Wab h' + bab) with Wab [128][128], and its bf16-rounded
 // values go through the head, Y3 = Wd2 g + bd2 (FP32 [M][3]); h' is not stored.
-template <bool DEC = false, int NWV = DEC ? 8 : kNodeWaves>   // DEC: 8 waves (more registers per wave, no spills)
+// FOLD: the B half of AB gets + Wc3 P of the row (NnFold; Wt [3][128] read per
+// tile through an opaque offset, so the compiler keeps it out of registers).
+template <bool DEC = false, int NWV = DEC ? 8 : kNodeWaves, bool FOLD = false>   // DEC: 8 waves (no spills)
 __global__ void __launch_bounds__(NWV * 64) k_node(const bf16_t *__restrict__ HS, int ldhs,
                                                           const bf16_t *__restrict__ W1, const float *__restrict__ b1,
                                                           const bf16_t *__restrict__ W2, const float *__restrict__ b2,
@@ -269,7 +271,7 @@ __global__ void __launch_bounds__(NWV * 64) k_node(const bf16_t *__restrict__ HS
                                                           const float *__restrict__ bab = nullptr,
                                                           const float *__restrict__ Wd2 = nullptr,
                                                           const float *__restrict__ bd2 = nullptr,
-                                                          float *__restrict__ Y3 = nullptr) {
+                                                          float *__restrict__ Y3 = nullptr, NnFold fold = NnFold{}) {
     constexpr int NAB = DEC ? 128 : 256;   // rows of the third weight
     extern __shared__ bf16_t wl[];
     bf16_t *w1 = wl, *w2 = wl + 128 * 256, *wab = w2 + 128 * 128;
@@ -400,6 +402,12 @@ __global__ void __launch_bounds__(NWV * 64) k_node(const bf16_t *__restrict__ HS
                 for (int o = 0; o < 3; o++) Y3[3 * (size_t)r + o] = part[o] + bd2[o];
         }
         // AB = Wab h', in two halves of 128 outputs
+        float px = 0.f, py = 0.f, pz = 0.f;
+        if (FOLD && r < M) {
+            px = (fold.pos[3 * (size_t)r] - fold.origin[0]) * fold.inv_r;
+            py = (fold.pos[3 * (size_t)r + 1] - fold.origin[1]) * fold.inv_r;
+            pz = (fold.pos[3 * (size_t)r + 2] - fold.origin[2]) * fold.inv_r;
+        }
 #pragma unroll
         for (int half = 0; half < (DEC ? 0 : 2); half++) {
 #pragma unroll
@@ -418,6 +426,16 @@ __global__ void __launch_bounds__(NWV * 64) k_node(const bf16_t *__restrict__ HS
             if (r < M)
 #pragma unroll
                 for (int c = 0; c < 8; c++) {
+                    if (FOLD && half == 1) {   // + Wc3 P of the row
+                        const int col = c * 16 + 4 * g + (lb - lr);
+                        const float4 wx = *reinterpret_cast<const float4 *>(fold.Wt + col);
+                        const float4 wy = *reinterpret_cast<const float4 *>(fold.Wt + 128 + col);
+                        const float4 wz = *reinterpret_cast<const float4 *>(fold.Wt + 256 + col);
+                        acc[c][0] += wx.x * px + wy.x * py + wz.x * pz;
+                        acc[c][1] += wx.y * px + wy.y * py + wz.y * pz;
+                        acc[c][2] += wx.z * px + wy.z * py + wz.z * pz;
+                        acc[c][3] += wx.w * px + wy.w * py + wz.w * pz;
+                    }
                     uint32_t o[4];
 #pragma unroll
                     for (int e = 0; e < 4; e++) o[e] = (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)acc[c][e]);
@@ -626,20 +644,36 @@ __global__ void __launch_bounds__(256) k_edge_eps(const float *__restrict__ pos,
 // BUF (rows of AB within 4 GB: N <= 8M nodes): B_j through one buffer
 // descriptor per wave, the row offset j * 512 as the load's scalar offset, so
 // a gather costs no vector instruction (else a 64-bit address add per edge).
-template <bool BUF>
+// order (NULL or a permutation of the nodes, e.g. the graph's cell order):
+// wave w sums node order[w], so the waves in flight read neighbouring rows of
+// B from L2 instead of HBM; the sums do not depend on it.
+// FOLD: B_j already holds + Wc3 P_j (P = (x - origin) / r), A_i gets - Wc3 P_i
+// here, so Wc3 (x_j - x_i) / r needs no work per edge: 1 add, 1 multiply-add
+// with the distance, max and add (4 instead of 7 instructions per edge).
+template <bool BUF, bool FOLD>
 __global__ void __launch_bounds__(256) k_edge_fwd_h(const h2_t *__restrict__ AB, const int *__restrict__ rp,
                                                     const int *__restrict__ ci, const uint2 *__restrict__ eps,
                                                     const float *__restrict__ Wc, const float *__restrict__ bc,
                                                     bf16_t *__restrict__ S, int lds, int N,
-                                                    const int *__restrict__ deg, int row_cap) {
-    const int i = __builtin_amdgcn_readfirstlane(xcd_block(gridDim.x) * 4 + (threadIdx.x >> 6));
-    if (i >= N) return;
+                                                    const int *__restrict__ deg, int row_cap,
+                                                    const int *__restrict__ order, NnFold fold) {
+    const int w = __builtin_amdgcn_readfirstlane(xcd_block(gridDim.x) * 4 + (threadIdx.x >> 6));
+    if (w >= N) return;
+    const int i = order ? __builtin_amdgcn_readfirstlane(order[w]) : w;
     const int lane = threadIdx.x & 63, f = 2 * lane;
     const float4 w0 = *reinterpret_cast<const float4 *>(Wc + 4 * f), w1 = *reinterpret_cast<const float4 *>(Wc + 4 * f + 4);
     const h2_t wx = {(_Float16)w0.x, (_Float16)w1.x}, wy = {(_Float16)w0.y, (_Float16)w1.y};
     const h2_t wz = {(_Float16)w0.z, (_Float16)w1.z}, wd = {(_Float16)w0.w, (_Float16)w1.w};
     const h2_t ai = AB[(size_t)i * 128 + lane];
-    const h2_t a2 = {(_Float16)((float)ai.x + bc[f]), (_Float16)((float)ai.y + bc[f + 1])};
+    float a0 = (float)ai.x + bc[f], a1 = (float)ai.y + bc[f + 1];
+    if constexpr (FOLD) {
+        const float px = (fold.pos[3 * i] - fold.origin[0]) * fold.inv_r;
+        const float py = (fold.pos[3 * i + 1] - fold.origin[1]) * fold.inv_r;
+        const float pz = (fold.pos[3 * i + 2] - fold.origin[2]) * fold.inv_r;
+        a0 -= w0.x * px + w0.y * py + w0.z * pz;
+        a1 -= w1.x * px + w1.y * py + w1.z * pz;
+    }
+    const h2_t a2 = {(_Float16)a0, (_Float16)a1};
     const h2_t zero = {(_Float16)0.f, (_Float16)0.f};
     const uint32_t lane_bytes = 4u * lane;
     const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
@@ -659,9 +693,11 @@ __global__ void __launch_bounds__(256) k_edge_fwd_h(const h2_t *__restrict__ AB,
     auto edge = [&](h2_t bj, uint2 e, h2_t acc) {
         const h2_t exy = bitsh2(e.x), ezd = bitsh2(e.y);
         h2_t c = a2 + bj;
-        c = __builtin_elementwise_fma(wx, __builtin_shufflevector(exy, exy, 0, 0), c);
-        c = __builtin_elementwise_fma(wy, __builtin_shufflevector(exy, exy, 1, 1), c);
-        c = __builtin_elementwise_fma(wz, __builtin_shufflevector(ezd, ezd, 0, 0), c);
+        if constexpr (!FOLD) {
+            c = __builtin_elementwise_fma(wx, __builtin_shufflevector(exy, exy, 0, 0), c);
+            c = __builtin_elementwise_fma(wy, __builtin_shufflevector(exy, exy, 1, 1), c);
+            c = __builtin_elementwise_fma(wz, __builtin_shufflevector(ezd, ezd, 0, 0), c);
+        }
         c = __builtin_elementwise_fma(wd, __builtin_shufflevector(ezd, ezd, 1, 1), c);
         return acc + __builtin_elementwise_maximum(c, zero);   // NaN-propagating: an overflow stays visible
     };
@@ -895,20 +931,25 @@ __global__ void k_g_rank(const int *key, const int *start, const int *order, con
 // when eps is given, the edge features of edge_eps), else the degree.
 // SLAB (FILL only): row i starts at i * row_cap instead of rp[i]; entries past
 // row_cap are not written, the degree is, and *over counts the rows cut.
+// Waves take the nodes in cell order (wave w: the node at ps[w]), so the waves
+// in flight on an XCD read the same few columns of cells from its L2; each row
+// is the same whichever wave writes it.
 template <bool FILL, bool SLAB = false>
 __global__ void __launch_bounds__(256) k_g_rows(const float *__restrict__ pos, const int *__restrict__ key,
                                                 const int *__restrict__ start, const float4 *__restrict__ ps, int N,
                                                 NnGrid gd, float r2, float inv_r, int *__restrict__ deg,
                                                 const int *__restrict__ rp, int *__restrict__ ci,
                                                 uint2 *__restrict__ eps, int row_cap = 0, int *over = nullptr) {
-    const int i = __builtin_amdgcn_readfirstlane(xcd_block(gridDim.x) * 4 + (threadIdx.x >> 6));
-    if (i >= N) return;
+    const int w = __builtin_amdgcn_readfirstlane(xcd_block(gridDim.x) * 4 + (threadIdx.x >> 6));
+    if (w >= N) return;
     const int lane = threadIdx.x & 63;
+    const float4 pw = ps[w];
+    const int i = __builtin_amdgcn_readfirstlane(__float_as_int(pw.w));
     const int k = key[i];
     const int sbase = k - k % gd.ncell;
     const int cell = k % gd.ncell;
     const int c[3] = {cell / (gd.cc[1] * gd.cc[2]), (cell / gd.cc[2]) % gd.cc[1], cell % gd.cc[2]};
-    const float xi = pos[3 * i], yi = pos[3 * i + 1], zi = pos[3 * i + 2];
+    const float xi = pw.x, yi = pw.y, zi = pw.z;
     const int z0 = max(c[2] - 1, 0), z1 = min(c[2] + 1, gd.cc[2] - 1);
     const int row0 = SLAB ? 0 : (FILL ? rp[i] : 0);
     int out = row0;
@@ -1192,17 +1233,20 @@ int dmph_nn_gemm_head(const uint16_t *X, int ldx, const uint16_t *W, const float
     return launch_ok();
 }
 
+}  // extern "C"
+
 // The encoder at inference, from the raw node data to the first round's input:
 // the 32 node features of k_features (computed per lane: lane (r, g) makes
 // features 8g .. 8g + 7 of row r, its MFMA fragment), h1 = relu(We1 x + be1),
 // h0 = We2 h1 + be2 (stored bf16 into H0, the left half of [h | S]) and
 // AB = Wab h0 (IEEE half), intermediates in registers (as k_node).
+template <bool FOLD>   // as k_node
 __global__ void __launch_bounds__(kNodeWaves * 64) k_enc(const float *__restrict__ vel_hist, const int *__restrict__ cls,
                                                          const float *__restrict__ pos, NnFeat ft,
                                                          const bf16_t *__restrict__ We1, const float *__restrict__ be1,
                                                          const bf16_t *__restrict__ We2, const float *__restrict__ be2,
                                                          const bf16_t *__restrict__ Wab, bf16_t *__restrict__ H0,
-                                                         int ldh0, bf16_t *__restrict__ ABo, int M) {
+                                                         int ldh0, bf16_t *__restrict__ ABo, int M, NnFold fold) {
     extern __shared__ bf16_t wl[];
     bf16_t *we1 = wl, *we2 = wl + 128 * 32, *wab = we2 + 128 * 128;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -1283,8 +1327,14 @@ __global__ void __launch_bounds__(kNodeWaves * 64) k_enc(const float *__restrict
             hf[c][1] = (uint32_t)f2bf(acc[c][2] + b.z) | (uint32_t)f2bf(acc[c][3] + b.w) << 16;
             if (r < M) *reinterpret_cast<uint2 *>(H0 + (size_t)r * ldh0 + c * 16 + 4 * g) = make_uint2(hf[c][0], hf[c][1]);
         }
+        float px = 0.f, py = 0.f, pz = 0.f;
 #pragma unroll
         for (int half = 0; half < 2; half++) {   // AB = Wab h0
+            if (FOLD && half == 1 && r < M) {
+                px = (pos[3 * (size_t)r] - fold.origin[0]) * fold.inv_r;
+                py = (pos[3 * (size_t)r + 1] - fold.origin[1]) * fold.inv_r;
+                pz = (pos[3 * (size_t)r + 2] - fold.origin[2]) * fold.inv_r;
+            }
 #pragma unroll
             for (int c = 0; c < 8; c++) acc[c] = f4_t{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
@@ -1301,6 +1351,16 @@ __global__ void __launch_bounds__(kNodeWaves * 64) k_enc(const float *__restrict
             if (r < M)
 #pragma unroll
                 for (int c = 0; c < 8; c++) {
+                    if (FOLD && half == 1) {   // + Wc3 P of the row
+                        const int col = c * 16 + 4 * g + (lb - lr);
+                        const float4 wx = *reinterpret_cast<const float4 *>(fold.Wt + col);
+                        const float4 wy = *reinterpret_cast<const float4 *>(fold.Wt + 128 + col);
+                        const float4 wz = *reinterpret_cast<const float4 *>(fold.Wt + 256 + col);
+                        acc[c][0] += wx.x * px + wy.x * py + wz.x * pz;
+                        acc[c][1] += wx.y * px + wy.y * py + wz.y * pz;
+                        acc[c][2] += wx.z * px + wy.z * py + wz.z * pz;
+                        acc[c][3] += wx.w * px + wy.w * py + wz.w * pz;
+                    }
                     uint32_t o[4];
 #pragma unroll
                     for (int e = 0; e < 4; e++) o[e] = (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)acc[c][e]);
@@ -1311,41 +1371,54 @@ __global__ void __launch_bounds__(kNodeWaves * 64) k_enc(const float *__restrict
     }
 }
 
+extern "C" {
+
 int dmph_nn_encode(const float *vel_hist, int N, const int *cls, const float *pos, const NnFeat *ft, const uint16_t *We1,
                    const float *be1, const uint16_t *We2, const float *be2, const uint16_t *Wab, uint16_t *H0, int ldh0,
-                   uint16_t *AB, void *stream) {
+                   uint16_t *AB, const NnFold *fold, void *stream) {
     if (N <= 0) return 0;
-    if (ldh0 % 4) return -1;
+    if (ldh0 % 4 || (fold && !fold->Wt)) return -1;
     constexpr size_t lds = (size_t)(128 * 32 + 128 * 128 + 256 * 128) * sizeof(bf16_t);   // 104 KB
-    static bool lds_set = false;
-    if (!lds_set) {
-        if (hipFuncSetAttribute(reinterpret_cast<const void *>(&k_enc), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)lds) != hipSuccess)
-            return -5;
-        lds_set = true;
+    static bool lds_set[2] = {false, false};
+    if (!lds_set[fold ? 1 : 0]) {
+        const void *k = fold ? reinterpret_cast<const void *>(&k_enc<true>) : reinterpret_cast<const void *>(&k_enc<false>);
+        if (hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess) return -5;
+        lds_set[fold ? 1 : 0] = true;
     }
     const unsigned tiles = (unsigned)((N + kNodeWaves * 16 - 1) / (kNodeWaves * 16));
     const unsigned grid = tiles < 256u ? tiles : 256u;
-    hipLaunchKernelGGL(k_enc, dim3(grid), dim3(kNodeWaves * 64), lds, (hipStream_t)stream, vel_hist, cls, pos, *ft, We1,
-                       be1, We2, be2, Wab, H0, ldh0, AB, N);
+    const NnFold fd = fold ? *fold : NnFold{};
+    if (fold)
+        hipLaunchKernelGGL(k_enc<true>, dim3(grid), dim3(kNodeWaves * 64), lds, (hipStream_t)stream, vel_hist, cls, pos,
+                           *ft, We1, be1, We2, be2, Wab, H0, ldh0, AB, N, fd);
+    else
+        hipLaunchKernelGGL(k_enc<false>, dim3(grid), dim3(kNodeWaves * 64), lds, (hipStream_t)stream, vel_hist, cls, pos,
+                           *ft, We1, be1, We2, be2, Wab, H0, ldh0, AB, N, fd);
     return launch_ok();
 }
 
 int dmph_nn_node(const uint16_t *HS, int ldhs, const uint16_t *W1, const float *b1, const uint16_t *W2, const float *b2,
-                 const uint16_t *Wab, uint16_t *Hn, int ldhn, uint16_t *AB, int M, void *stream) {
+                 const uint16_t *Wab, uint16_t *Hn, int ldhn, uint16_t *AB, int M, const NnFold *fold, void *stream) {
     if (M <= 0) return 0;
-    if (ldhs % 8 || ldhn % 4) return -1;
-    static bool lds_set = false;   // the launch needs all 160 KB of LDS
-    if (!lds_set) {
-        if (hipFuncSetAttribute(reinterpret_cast<const void *>(&k_node<false>),
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)kNodeLds) != hipSuccess)
-            return -5;
-        lds_set = true;
+    if (ldhs % 8 || ldhn % 4 || (fold && (!fold->pos || !fold->Wt))) return -1;
+    static bool lds_set[2] = {false, false};   // the launch needs all 160 KB of LDS
+    if (!lds_set[fold ? 1 : 0]) {
+        const void *k = fold ? reinterpret_cast<const void *>(&k_node<false, kNodeWaves, true>)
+                             : reinterpret_cast<const void *>(&k_node<false, kNodeWaves, false>);
+        if (hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kNodeLds) != hipSuccess) return -5;
+        lds_set[fold ? 1 : 0] = true;
     }
     const unsigned tiles = (unsigned)((M + kNodeWaves * 16 - 1) / (kNodeWaves * 16));
     const unsigned grid = tiles < 256u ? tiles : 256u;   // one resident workgroup per CU
-    hipLaunchKernelGGL(k_node<false>, dim3(grid), dim3(kNodeWaves * 64), kNodeLds, (hipStream_t)stream, HS, ldhs, W1,
-                       b1, W2, b2, Wab, Hn, ldhn, AB, M, nullptr, nullptr, nullptr, nullptr);
+    const NnFold fd = fold ? *fold : NnFold{};
+    if (fold)
+        hipLaunchKernelGGL((k_node<false, kNodeWaves, true>), dim3(grid), dim3(kNodeWaves * 64), kNodeLds,
+                           (hipStream_t)stream, HS, ldhs, W1, b1, W2, b2, Wab, Hn, ldhn, AB, M, nullptr, nullptr, nullptr,
+                           nullptr, fd);
+    else
+        hipLaunchKernelGGL((k_node<false, kNodeWaves, false>), dim3(grid), dim3(kNodeWaves * 64), kNodeLds,
+                           (hipStream_t)stream, HS, ldhs, W1, b1, W2, b2, Wab, Hn, ldhn, AB, M, nullptr, nullptr, nullptr,
+                           nullptr, fd);
     return launch_ok();
 }
 
@@ -1413,18 +1486,22 @@ int dmph_nn_edge_eps(const float *pos, const int *rp, const int *ci, uint32_t *e
 }
 
 int dmph_nn_edge_fwd_f16(const uint16_t *AB, const int *rp, const int *ci, const uint32_t *eps, const float *Wc,
-                         const float *bc, uint16_t *S, int lds, int N, const int *deg, int row_cap, void *stream) {
+                         const float *bc, uint16_t *S, int lds, int N, const int *deg, int row_cap,
+                         const int *order, const NnFold *fold, void *stream) {
     if (N <= 0) return 0;
     if (lds < 128 || lds % 2 || (deg && (row_cap <= 0 || (long long)N * row_cap > 0x7fffffffLL))) return -1;
+    if (fold && !fold->pos) return -1;
     const bool buf = (size_t)N * 512 <= 0xffffffffull - 1024;   // row offsets fit the 32-bit scalar offset
+    const NnFold fd = fold ? *fold : NnFold{};
+    auto go = [&](auto kern) {
+        hipLaunchKernelGGL(kern, dim3((N + 3) / 4), dim3(256), 0, (hipStream_t)stream,
+                           reinterpret_cast<const h2_t *>(AB), rp, ci, reinterpret_cast<const uint2 *>(eps), Wc, bc, S,
+                           lds, N, deg, row_cap, order, fd);
+    };
     if (buf)
-        hipLaunchKernelGGL(k_edge_fwd_h<true>, dim3((N + 3) / 4), dim3(256), 0, (hipStream_t)stream,
-                           reinterpret_cast<const h2_t *>(AB), rp, ci, reinterpret_cast<const uint2 *>(eps), Wc, bc, S,
-                           lds, N, deg, row_cap);
+        fold ? go(k_edge_fwd_h<true, true>) : go(k_edge_fwd_h<true, false>);
     else
-        hipLaunchKernelGGL(k_edge_fwd_h<false>, dim3((N + 3) / 4), dim3(256), 0, (hipStream_t)stream,
-                           reinterpret_cast<const h2_t *>(AB), rp, ci, reinterpret_cast<const uint2 *>(eps), Wc, bc, S,
-                           lds, N, deg, row_cap);
+        fold ? go(k_edge_fwd_h<false, true>) : go(k_edge_fwd_h<false, false>);
     return launch_ok();
 }
 

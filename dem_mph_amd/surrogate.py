@@ -52,6 +52,16 @@ class NnFeat(C.Structure):
     _fields_ = [("vscale", C.c_float), ("tlo", C.c_float * 3), ("thi", C.c_float * 3), ("inv_r", C.c_float)]
 
 
+class NnFold(C.Structure):
+    _fields_ = [("pos", C.c_void_p), ("Wt", C.c_void_p), ("origin", C.c_float * 3), ("inv_r", C.c_float)]
+
+
+# The position fold (dmph_nn_edge_fwd_f16) rounds Wc3 (x - origin) / radius to
+# half in the node terms: used while the nodes' box reaches at most this many
+# radii from its centre (the C3 tank: 14), else the edge sums take the edge
+# features whole.
+FOLD_MAX_RADII = 16.0
+
 _nn = None
 
 
@@ -145,7 +155,10 @@ class Surrogate:
         self._graph_ws = None
         self.graph_eps = None   # edge features of the last graph(..., eps=True)
         self._bbox = None
+        self._box = None        # host copy of the last graph's bounding box, with the positions it belongs to
         self._folded = None   # inference: per round [n1 left | n1 right @ w2] (rebuilt after a weight change)
+        self._wt = None       # inference: per round Wc[:, 0:3] transposed (the position fold)
+        self.pos_fold = True  # fold the edge features' linear part into the node terms where the box allows
 
     def at_spacing(self, d: float) -> "Surrogate":
         """Use the network on particles of spacing d: the graph radius becomes
@@ -199,6 +212,7 @@ class Surrogate:
                 l.b.copy_(self.torch.tensor(sd[name + ".b"], dtype=self.torch.float32))
         self._refresh_bf16()
         self._folded = None
+        self._wt = None
 
     # ------------------------------------------------------------------ graph
     def graph(self, pos, scene=None, nscene: int = 1, eps: bool = False) -> Tuple[object, object, int]:
@@ -243,6 +257,7 @@ class Surrogate:
                           torch.empty(lib.dmph_nn_bbox_scratch(N) // 4 + 1, dtype=torch.float32, device=self.dev))
         _chk(lib.dmph_nn_bbox(_p(pos), N, _p(self._bbox[0]), _p(self._bbox[1]), stream), "dmph_nn_bbox")
         box = self._bbox[0].cpu().numpy().astype(np.float64)
+        self._box = (box, pos.data_ptr(), N)
         lo, hi = box[:3] - 1e-6, box[3:] + 1e-6
         w = self.radius * (1.0 + 1e-5)            # cubic cells at least the radius wide
         cc = np.maximum(np.ceil((hi - lo) / w).astype(int), 1)
@@ -381,9 +396,35 @@ class Surrogate:
                                       C.c_float(0.0), _p(f.wb), _p(f.wbt), f.n_out, f.n_in, self._s()), "bf16 copy")
                 out.append(f)
             self._folded = out
+            self._wt = [L["c"].w[:, :3].t().contiguous() for L in self.rounds]
         return self._folded
 
-    def forward(self, x0, pos, rp, ci, N: int, train: bool = True, eps=None, rows=None, feats=None):
+    def _fold_args(self, pos, N: int):
+        """NnFold for these positions (origin at the centre of their box), or
+        None where the box is unknown or too wide for the fold's half rounding."""
+        if not self.pos_fold or self._box is None:
+            return None
+        box, ptr, n = self._box
+        if ptr != pos.data_ptr() or n != N:
+            return None
+        lo, hi = box[:3], box[3:]
+        if float(np.max(hi - lo)) * 0.5 / self.radius > FOLD_MAX_RADII:
+            return None
+        f = NnFold()
+        f.pos = pos.data_ptr()
+        for k in range(3):
+            f.origin[k] = float(0.5 * (lo[k] + hi[k]))
+        f.inv_r = 1.0 / self.radius
+        return f
+
+    def _fold_for(self, f, r: int):
+        """f with the transposed Wc[:, 0:3] of round r (None stays None)."""
+        if f is None:
+            return None
+        f.Wt = self._wt[r].data_ptr()
+        return C.byref(f)
+
+    def forward(self, x0, pos, rp, ci, N: int, train: bool = True, eps=None, rows=None, feats=None, order=None):
         """Accelerations (scaled) y [N, 3] FP32; with `train` every round's
         activations are kept for backward, else two slots alternate and the
         aggregation's linear layer is folded into the node MLP (_fold)."""
@@ -394,15 +435,17 @@ class Surrogate:
         slot = (lambda r: r) if train else (lambda r: 0)
         hslot = (lambda r: r) if train else (lambda r: r % 2)
         fused = not train and self.kstep == 32
+        folded = None if train else self._fold()
+        # the position fold needs every [A | B] from a fused kernel (encoder included)
+        fold = self._fold_args(pos, N) if fused and feats is not None else None
         if fused and feats is not None:   # features, encoder and the first [A | B] in one kernel
             vel_hist, cls = feats
             _chk(lib.dmph_nn_encode(_p(vel_hist), N, _p(cls), _p(pos), C.byref(self._feat()), _p(self.enc1.wb),
                                     _p(self.enc1.b), _p(self.enc2.wb), _p(self.enc2.b), _p(self.rounds[0]["ab"].wb),
-                                    _p(ws["hin"][0]), 2 * W, _p(ws["ab"][0]), s), "encode")
+                                    _p(ws["hin"][0]), 2 * W, _p(ws["ab"][0]), self._fold_for(fold, 0), s), "encode")
         else:
             self._gemm(x0, FEAT, self.enc1, ws["h1"], W, N, True)
             self._gemm(ws["h1"], W, self.enc2, ws["hin"][0], 2 * W, N, False)
-        folded = None if train else self._fold()
         if not train and eps is None:   # edge features once per graph, for the half-precision edge sums
             if ws.get("eps") is None or ws["eps"].numel() < 2 * ci.numel():
                 ws["eps"] = self.torch.empty(2 * ci.numel(), dtype=self.torch.int32, device=self.dev)
@@ -417,16 +460,18 @@ class Surrogate:
             if fused and r + 1 < ROUNDS:
                 _chk(lib.dmph_nn_edge_fwd_f16(_p(ab), _p(rp), _p(ci), _p(eps), _p(L["c"].w), _p(L["c"].b),
                                               _p(hin[:, W:]), 2 * W, N, _p(rows[0] if rows else None),
-                                              rows[1] if rows else 0, s), "edge_fwd_f16")
+                                              rows[1] if rows else 0, _p(order), self._fold_for(fold, r), s),
+                     "edge_fwd_f16")
                 nxt = self.rounds[r + 1]["ab"]
                 _chk(lib.dmph_nn_node(_p(hin), 2 * W, _p(folded[r].wb), _p(folded[r].b), _p(L["n2"].wb),
-                                      _p(L["n2"].b), _p(nxt.wb), _p(ws["hin"][hslot(r + 1)]), 2 * W, _p(ab), N, s),
-                     "node")
+                                      _p(L["n2"].b), _p(nxt.wb), _p(ws["hin"][hslot(r + 1)]), 2 * W, _p(ab), N,
+                                      self._fold_for(fold, r + 1), s), "node")
                 continue
             if fused:   # the last round and the decoder in one kernel
                 _chk(lib.dmph_nn_edge_fwd_f16(_p(ab), _p(rp), _p(ci), _p(eps), _p(L["c"].w), _p(L["c"].b),
                                               _p(hin[:, W:]), 2 * W, N, _p(rows[0] if rows else None),
-                                              rows[1] if rows else 0, s), "edge_fwd_f16")
+                                              rows[1] if rows else 0, _p(order), self._fold_for(fold, r), s),
+                     "edge_fwd_f16")
                 _chk(lib.dmph_nn_node_head(_p(hin), 2 * W, _p(folded[r].wb), _p(folded[r].b), _p(L["n2"].wb),
                                            _p(L["n2"].b), _p(self.dec1.wb), _p(self.dec1.b), _p(self.dec2.w),
                                            _p(self.dec2.b), _p(ws["y"]), N, s), "node_head")
@@ -440,7 +485,8 @@ class Surrogate:
             else:   # S straight into the right half of hin_r, then the folded node layer
                 deg, cap = rows if rows is not None else (None, 0)
                 _chk(lib.dmph_nn_edge_fwd_f16(_p(ab), _p(rp), _p(ci), _p(eps), _p(L["c"].w), _p(L["c"].b),
-                                              _p(hin[:, W:]), 2 * W, N, _p(deg), cap, s), "edge_fwd_f16")
+                                              _p(hin[:, W:]), 2 * W, N, _p(deg), cap, _p(order), None, s),
+                     "edge_fwd_f16")
                 self._gemm(hin, 2 * W, folded[r], u, W, N, True)
             self._gemm(u, W, L["n2"], ws["hin"][hslot(r + 1)], 2 * W, N, False, R=hin, ldr=2 * W)
         if train:
@@ -518,11 +564,13 @@ class Surrogate:
         fused = self.kstep == 32
         x0 = None if fused else self.features(vel_hist, cls, pos, train=False)
         feats = (vel_hist, cls) if fused else None
-        if r is not None:
+        if r is not None:   # edge sums in the graph's cell order (its `order` work array)
             ci, deg, cap, eps = r
-            return self.forward(x0, pos, None, ci, N, train=False, eps=eps, rows=(deg, cap), feats=feats)
+            return self.forward(x0, pos, None, ci, N, train=False, eps=eps, rows=(deg, cap), feats=feats,
+                                order=self._graph_ws["order"])
         rp, ci, _ = self.graph(pos, scene, nscene, eps=True)
-        return self.forward(x0, pos, rp, ci, N, train=False, eps=self.graph_eps, feats=feats)
+        return self.forward(x0, pos, rp, ci, N, train=False, eps=self.graph_eps, feats=feats,
+                            order=self._graph_ws["order"])
 
     def rollout_step(self, pos, vel_hist, y, cls, body, nbodies: int, h: float, body_sums) -> None:
         """x, v history advanced by stride h from the network's output y."""

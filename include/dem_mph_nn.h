@@ -34,6 +34,22 @@ typedef struct NnFeat {
     float inv_r;
 } NnFeat;
 
+/* position fold of the inference edge sums.  The linear part of Wc eps_ij,
+ * Wc[:, 0:3] (x_j - x_i) / radius, equals Wc3 P_j - Wc3 P_i with
+ * P = (x - origin) * inv_r, so it moves into the node terms: the kernel that
+ * writes AB adds Wc3 P_j to the B half, and dmph_nn_edge_fwd_f16 subtracts
+ * Wc3 P_i from A_i, which leaves one multiply-add per edge (the distance).
+ * Exact in real arithmetic; in IEEE half the B and A terms round at about
+ * 2^-11 |Wc3 P|, so the origin should be the centre of the nodes' box and the
+ * box a few tens of radii across at most. */
+typedef struct NnFold {
+    const float *pos;   /* node positions [N][3] */
+    const float *Wt;    /* [3][128]: Wc[:, 0:3] transposed, of the edge layer that reads the AB being
+                           written (dmph_nn_encode, dmph_nn_node; unused by dmph_nn_edge_fwd_f16) */
+    float origin[3];
+    float inv_r;        /* 1 / radius */
+} NnFold;
+
 /* gemm flags */
 #define DMPH_NN_RELU 1      /* ReLU after bias and residual */
 #define DMPH_NN_F16_OUT 2   /* Y in IEEE half instead of bf16 */
@@ -54,16 +70,18 @@ int dmph_nn_gemm_head(const uint16_t *X, int ldx, const uint16_t *W, const float
  * u = relu(W1 HS + b1) (HS [M][256] = [h | S], row stride ldhs; W1 [128][256]),
  * Hn = h + W2 u + b2 (bf16, [M][128] at row stride ldhn; W2 [128][128]),
  * AB = Wab Hn (IEEE half [M][256]; Wab [256][128]).  Equal to dmph_nn_gemm three
- * times up to the order of FP32 sums.  Uses 160 KB of LDS per workgroup. */
+ * times up to the order of FP32 sums.  fold (NULL: none): B gets + Wc3 P (NnFold).
+ * Uses 160 KB of LDS per workgroup. */
 int dmph_nn_node(const uint16_t *HS, int ldhs, const uint16_t *W1, const float *b1, const uint16_t *W2, const float *b2,
-                 const uint16_t *Wab, uint16_t *Hn, int ldhn, uint16_t *AB, int M, void *stream);
+                 const uint16_t *Wab, uint16_t *Hn, int ldhn, uint16_t *AB, int M, const NnFold *fold, void *stream);
 
 /* the encoder at inference in one kernel: the features of dmph_nn_features,
  * then H0 = We2 relu(We1 x + be1) + be2 (bf16, row stride ldh0; We1 [128][32],
- * We2 [128][128]) and AB = Wab H0 (IEEE half [N][256]; Wab [256][128]) */
+ * We2 [128][128]) and AB = Wab H0 (IEEE half [N][256]; Wab [256][128]); fold as
+ * dmph_nn_node */
 int dmph_nn_encode(const float *vel_hist, int N, const int *cls, const float *pos, const NnFeat *ft, const uint16_t *We1,
                    const float *be1, const uint16_t *We2, const float *be2, const uint16_t *Wab, uint16_t *H0, int ldh0,
-                   uint16_t *AB, void *stream);
+                   uint16_t *AB, const NnFold *fold, void *stream);
 
 /* the last round's node update and the decoder in one kernel at inference:
  * h' as dmph_nn_node (not stored), then Y3 [M][3] = Wd2 relu(Wd1 h' + bd1) + bd2
@@ -92,10 +110,18 @@ int dmph_nn_edge_fwd(const uint16_t *AB, const float *pos, const int *rp, const 
  * half ([N][256], DMPH_NN_F16_OUT), packed half arithmetic in batches of 16
  * edges added into FP32 sums; S as in edge_fwd (bf16).  With deg (not NULL) the
  * rows are those of dmph_nn_graph_rows: row i at i * row_cap, deg[i] entries
- * (rp unused), edge_eps not needed (the graph writes eps). */
+ * (rp unused), edge_eps not needed (the graph writes eps).  With fold (not
+ * NULL) the B half of AB holds + Wc3 P_j (written by a folding dmph_nn_node or
+ * dmph_nn_encode with the same origin and inv_r), A_i gets - Wc3 P_i here and
+ * only the distance of eps is read: one packed multiply-add per edge instead of four.
+ * order (NULL or a permutation of [0, N), e.g. the `order` work array of the
+ * graph builders, which lists the nodes by cell): the nodes are summed in that
+ * order, so the rows of B read at one time are neighbours in space and come
+ * from L2; the sums are the same. */
 int dmph_nn_edge_eps(const float *pos, const int *rp, const int *ci, uint32_t *eps, int N, float radius, void *stream);
 int dmph_nn_edge_fwd_f16(const uint16_t *AB, const int *rp, const int *ci, const uint32_t *eps, const float *Wc,
-                         const float *bc, uint16_t *S, int lds, int N, const int *deg, int row_cap, void *stream);
+                         const float *bc, uint16_t *S, int lds, int N, const int *deg, int row_cap,
+                         const int *order, const NnFold *fold, void *stream);
 /* gradients of edge_fwd given dS [N][128] FP32: dAB [N][256] bf16, dWc, dbc (+)=;
  * the graph must be symmetric; scratch of dmph_nn_edge_scratch bytes */
 size_t dmph_nn_edge_scratch(int N);

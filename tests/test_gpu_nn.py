@@ -124,7 +124,8 @@ def test_node_update_matches_three_gemms(S, M):
     assert lib.dmph_nn_gemm(_p(h1), 128, _p(Wabd), None, None, 0, _p(ab1), 256, M, 128, 256, S.GEMM_F16_OUT, 32, s) == 0
     h2 = torch.zeros((M, 256), dtype=torch.int16, device="cuda:0")   # h' into the left half of a [h | S] row
     ab2 = torch.full((M, 256), -1, dtype=torch.int16, device="cuda:0")
-    assert lib.dmph_nn_node(_p(HSd), 256, _p(W1d), _p(b1), _p(W2d), _p(b2), _p(Wabd), _p(h2), 256, _p(ab2), M, s) == 0
+    assert lib.dmph_nn_node(_p(HSd), 256, _p(W1d), _p(b1), _p(W2d), _p(b2), _p(Wabd), _p(h2), 256, _p(ab2), M, None,
+                            s) == 0
     a, b = _host_bf16(h1), _host_bf16(h2[:, :128])
     assert not h2[:, 128:].any()
     assert np.abs(a - b).max() <= 2e-2 * np.abs(a).max() and np.abs(a - b).mean() <= 1e-3 * np.abs(a).mean()
@@ -184,7 +185,7 @@ def test_encode_matches_features_and_gemms(S):
     h0b = torch.zeros((n, 256), dtype=torch.int16, device="cuda:0")
     ab2 = torch.zeros((n, 256), dtype=torch.int16, device="cuda:0")
     assert lib.dmph_nn_encode(_p(vh), n, _p(cls), _p(pos), C.byref(sur._feat()), _p(sur.enc1.wb), _p(sur.enc1.b),
-                              _p(sur.enc2.wb), _p(sur.enc2.b), _p(L0.wb), _p(h0b), 256, _p(ab2), s) == 0
+                              _p(sur.enc2.wb), _p(sur.enc2.b), _p(L0.wb), _p(h0b), 256, _p(ab2), None, s) == 0
     a, b = _host_bf16(h0[:, :128]), _host_bf16(h0b[:, :128])
     assert np.abs(a - b).max() <= 2e-2 * np.abs(a).max() and np.abs(a - b).mean() <= 1e-3 * np.abs(a).mean()
     x = ab1.cpu().numpy().view(np.float16).astype(np.float64)
@@ -262,10 +263,17 @@ def test_fixed_capacity_rows_equal_the_csr_graph(S):
     o1 = torch.zeros((n, 128), dtype=torch.int16, device="cuda:0")
     o2 = torch.zeros((n, 128), dtype=torch.int16, device="cuda:0")
     assert lib.dmph_nn_edge_fwd_f16(_p(abd), _p(rp), _p(ci), _p(sur.graph_eps), _p(Wc), _p(bc), _p(o1), 128, n,
-                                    None, 0, s) == 0
+                                    None, 0, None, None, s) == 0
     assert lib.dmph_nn_edge_fwd_f16(_p(abd), None, _p(ci2), _p(eps2), _p(Wc), _p(bc), _p(o2), 128, n, _p(deg), cap,
-                                    s) == 0
+                                    None, None, s) == 0
     assert torch.equal(o1, o2)
+    # summed in the graph's cell order: the same sums
+    order = sur._graph_ws["order"][:n]
+    assert np.array_equal(np.sort(order.cpu().numpy()), np.arange(n))
+    o3 = torch.zeros((n, 128), dtype=torch.int16, device="cuda:0")
+    assert lib.dmph_nn_edge_fwd_f16(_p(abd), None, _p(ci2), _p(eps2), _p(Wc), _p(bc), _p(o3), 128, n, _p(deg), cap,
+                                    _p(order), None, s) == 0
+    assert torch.equal(o1, o3)
 
 
 def test_radius_graph_refuses_more_than_int32_edges(S):
@@ -355,7 +363,7 @@ def test_half_edge_sums_match_fp64(S):
     out = torch.zeros((n, 256), dtype=torch.int16, device="cuda:0")
     assert lib.dmph_nn_edge_eps(_p(pos), _p(rp), _p(ci), _p(eps), n, C.c_float(sur.radius), s) == 0
     assert lib.dmph_nn_edge_fwd_f16(_p(abd), _p(rp), _p(ci), _p(eps), _p(Wcd), _p(bcd), _p(out[:, 128:]), 256, n,
-                                    None, 0, s) == 0
+                                    None, 0, None, None, s) == 0
     got = _bf16_float(out.cpu().numpy().view(np.uint16)[:, 128:])
     assert not out[:, :128].any()   # the left half untouched
     P = pos.cpu().numpy().astype(np.float64)
@@ -370,6 +378,146 @@ def test_half_edge_sums_match_fp64(S):
     err = np.abs(got - ref)
     assert err.max() <= 1e-2 * np.abs(ref).max(), err.max()
     assert np.linalg.norm(got - ref) <= 4e-3 * np.linalg.norm(ref)
+
+
+def _fold_arg(S, pos_t, origin, inv_r, Wt=None):
+    f = S.NnFold()
+    f.pos = pos_t.data_ptr()
+    f.Wt = Wt.data_ptr() if Wt is not None else 0
+    for k in range(3):
+        f.origin[k] = float(origin[k])
+    f.inv_r = float(inv_r)
+    return f
+
+
+@pytest.mark.parametrize("reach", [0.0, 12.0])
+def test_folded_half_edge_sums_match_fp64(S, reach):
+    """Position fold: with B_j + Wc3 P_j (P = (x - origin) / r, rounded to half
+    on the host) and the fold passed, the edge kernel subtracts Wc3 P_i from A_i
+    and multiplies only the distance per edge.  The sums equal the FP64 sums of
+    the unfolded formula to the unfolded kernel's tolerance, also with the
+    origin moved so that |P| reaches 16 radii (the driver's limit); rows of
+    every length as in the unfolded test."""
+    sur, sc, pos, vh, cls, tgt, mask = _small_problem(S, seed=3)
+    n = sc.n
+    rp0, ci0, nnz0 = sur.graph(pos)
+    RP0, CI0 = rp0.cpu().numpy(), ci0.cpu().numpy()
+    lens = np.minimum(np.diff(RP0), np.arange(n) % 71)
+    RP = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    CI = np.concatenate([CI0[RP0[i]:RP0[i] + lens[i]] for i in range(n)]).astype(np.int32)
+    rp, ci = torch.tensor(RP, device="cuda:0"), torch.tensor(CI, device="cuda:0")
+    rng = np.random.default_rng(3)
+    ab = (rng.standard_normal((n, 256)) * 0.7).astype(np.float16)
+    Wc = (rng.standard_normal((128, 4)) * 0.8).astype(np.float32)
+    bc = (rng.standard_normal(128) * 0.3).astype(np.float32)
+    P = pos.cpu().numpy().astype(np.float64)
+    r = sur.radius
+    origin = 0.5 * (P.min(0) + P.max(0)) - np.array([reach * r, 0.0, 0.0])
+    Pn = (P - origin) / r
+    assert np.abs(Pn).max() > (15.0 if reach else 4.0)
+    abf = ab.copy()
+    abf[:, 128:] = (ab[:, 128:].astype(np.float64) + Pn @ Wc[:, :3].T.astype(np.float64)).astype(np.float16)
+    lib = S.nnlib()
+    s = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    abd = torch.tensor(abf.view(np.int16), device="cuda:0")
+    Wcd, bcd = torch.tensor(Wc, device="cuda:0"), torch.tensor(bc, device="cuda:0")
+    eps = torch.empty(2 * ci.numel(), dtype=torch.int32, device="cuda:0")
+    out = torch.zeros((n, 128), dtype=torch.int16, device="cuda:0")
+    fd = _fold_arg(S, pos, origin, 1.0 / r)
+    assert lib.dmph_nn_edge_eps(_p(pos), _p(rp), _p(ci), _p(eps), n, C.c_float(r), s) == 0
+    assert lib.dmph_nn_edge_fwd_f16(_p(abd), _p(rp), _p(ci), _p(eps), _p(Wcd), _p(bcd), _p(out), 128, n,
+                                    None, 0, None, C.byref(fd), s) == 0
+    got = _bf16_float(out.cpu().numpy().view(np.uint16))
+    A, B = ab[:, :128].astype(np.float64), ab[:, 128:].astype(np.float64)
+    ref = np.zeros((n, 128))
+    for i in range(n):
+        J = CI[RP[i]:RP[i + 1]]
+        d = P[J] - P[i]
+        e = np.concatenate([d, np.linalg.norm(d, axis=1, keepdims=True)], axis=1) / r
+        ref[i] = np.maximum(A[i] + bc + B[J] + e @ Wc.T.astype(np.float64), 0.0).sum(axis=0)
+    err = np.abs(got - ref)
+    assert err.max() <= 1e-2 * np.abs(ref).max(), err.max()
+    assert np.linalg.norm(got - ref) <= 4e-3 * np.linalg.norm(ref)
+
+
+def _check_fold(ab_u, ab_f, WP):
+    """A half bit for bit; B half = unfolded + Wc3 P to the half rounding of both"""
+    assert torch.equal(ab_u[:, :128], ab_f[:, :128])
+    u = ab_u[:, 128:].cpu().numpy().view(np.float16).astype(np.float64)
+    f = ab_f[:, 128:].cpu().numpy().view(np.float16).astype(np.float64)
+    assert np.isfinite(f).all()
+    assert (np.abs(f - (u + WP)) <= 2.0 ** -10 * (np.abs(u) + np.abs(WP)) + 1e-4).all()
+    assert np.abs(WP).max() > 1.0   # the term is there
+
+
+def test_node_and_encode_fold_add_the_position_term(S):
+    """dmph_nn_node and dmph_nn_encode with a fold write the same h' and A
+    half as without, and B + Wc3 (x - origin) / r of each row (Wt = Wc3
+    transposed), to half rounding."""
+    lib = S.nnlib()
+    s = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    rng = np.random.default_rng(12)
+    Wc = (rng.standard_normal((128, 4)) * 0.8).astype(np.float32)
+    Wt = torch.tensor(np.ascontiguousarray(Wc[:, :3].T), device="cuda:0")
+    # node update
+    M = 3001
+    HSd = _dev_bf16(rng.standard_normal((M, 256)).astype(np.float32))
+    W1d = _dev_bf16((rng.standard_normal((128, 256)) / 16).astype(np.float32))
+    W2d = _dev_bf16((rng.standard_normal((128, 128)) / 11).astype(np.float32))
+    Wabd = _dev_bf16((rng.standard_normal((256, 128)) / 11).astype(np.float32))
+    b1, b2 = (torch.tensor(rng.standard_normal(128).astype(np.float32), device="cuda:0") for _ in range(2))
+    P = rng.uniform(0.0, 1.0, (M, 3)).astype(np.float32)
+    pos = torch.tensor(P, device="cuda:0")
+    origin, r = np.array([0.5, 0.4, 0.6]), 0.06
+    fd = _fold_arg(S, pos, origin, 1.0 / r, Wt)
+    outs = []
+    for f in (None, C.byref(fd)):
+        h = torch.zeros((M, 256), dtype=torch.int16, device="cuda:0")
+        ab = torch.full((M, 256), -1, dtype=torch.int16, device="cuda:0")
+        assert lib.dmph_nn_node(_p(HSd), 256, _p(W1d), _p(b1), _p(W2d), _p(b2), _p(Wabd), _p(h), 256, _p(ab), M, f,
+                                s) == 0
+        outs.append((h, ab))
+    assert torch.equal(outs[0][0], outs[1][0])
+    _check_fold(outs[0][1], outs[1][1], ((P.astype(np.float64) - origin) / r) @ Wc[:, :3].T.astype(np.float64))
+    # encoder
+    sur, sc, pos, vh, cls, tgt, mask = _small_problem(S, seed=5)
+    n = sc.n
+    L0 = sur.rounds[0]["ab"]
+    Pn = pos.cpu().numpy().astype(np.float64)
+    origin = 0.5 * (Pn.min(0) + Pn.max(0))
+    fd = _fold_arg(S, pos, origin, 1.0 / sur.radius, Wt)
+    outs = []
+    for f in (None, C.byref(fd)):
+        h0 = torch.zeros((n, 256), dtype=torch.int16, device="cuda:0")
+        ab = torch.zeros((n, 256), dtype=torch.int16, device="cuda:0")
+        assert lib.dmph_nn_encode(_p(vh), n, _p(cls), _p(pos), C.byref(sur._feat()), _p(sur.enc1.wb),
+                                  _p(sur.enc1.b), _p(sur.enc2.wb), _p(sur.enc2.b), _p(L0.wb), _p(h0), 256, _p(ab), f,
+                                  s) == 0
+        outs.append((h0, ab))
+    assert torch.equal(outs[0][0], outs[1][0])
+    _check_fold(outs[0][1], outs[1][1], ((Pn - origin) / sur.radius) @ Wc[:, :3].T.astype(np.float64))
+
+
+def test_inference_with_position_fold_matches_fp64_oracle(S):
+    """The fused inference path (encoder, node kernels, fixed-capacity rows)
+    with the position fold active: outputs within 5% (norm) of the FP64
+    oracle and within 1% of the same path without the fold."""
+    sur, sc, pos, vh, cls, tgt, mask = _small_problem(S, seed=6)
+    n = sc.n
+    rp, ci, nnz = sur.graph(pos)
+    X0 = _host_bf16(sur.features(vh, cls, pos, train=False)[:n])
+    yr, _ = NO.forward(sur.state_dict(), X0, pos.cpu().numpy().astype(np.float64), rp.cpu().numpy(),
+                       ci.cpu().numpy()[:nnz], sur.radius)
+    ys = []
+    for on in (True, False):
+        sur.pos_fold = on
+        ci2, deg, cap, eps2 = sur.graph_rows(pos)
+        assert (sur._fold_args(pos, n) is not None) == on
+        ys.append(sur.forward(None, pos, None, ci2, n, train=False, eps=eps2, rows=(deg, cap),
+                              feats=(vh, cls), order=sur._graph_ws["order"]).cpu().numpy().copy())
+    assert np.isfinite(ys[0]).all()
+    assert np.linalg.norm(ys[0] - yr) / np.linalg.norm(yr) < 0.05
+    assert np.linalg.norm(ys[0] - ys[1]) / np.linalg.norm(ys[1]) < 0.01
 
 
 def test_inference_forward_matches_fp64_oracle(S):
