@@ -714,6 +714,7 @@ __global__ void __launch_bounds__(256) k_edge_fwd_h(const h2_t *__restrict__ AB,
         s1 += (float)acc.y;
         q += B;
     };
+    // (32 edges per round trip measured no faster)
     while (q + kEdgeBatch <= q1) batch(std::integral_constant<int, kEdgeBatch>{});
     // the rest (< 16) in batches of 8, 4, 2, 1: at most four round trips
     if (q + 8 <= q1) batch(std::integral_constant<int, 8>{});

@@ -427,7 +427,11 @@ class Surrogate:
     def forward(self, x0, pos, rp, ci, N: int, train: bool = True, eps=None, rows=None, feats=None, order=None):
         """Accelerations (scaled) y [N, 3] FP32; with `train` every round's
         activations are kept for backward, else two slots alternate and the
-        aggregation's linear layer is folded into the node MLP (_fold)."""
+        aggregation's linear layer is folded into the node MLP (_fold).
+        Inference with `feats` (vel_hist, cls) runs the fused kernels (encoder,
+        node updates, position fold where the box allows); `order` (a
+        permutation, e.g. the graph's cell order) is the order the edge sums
+        visit the nodes in."""
         lib = nnlib()
         ws = self._workspace(N, train)
         W = WIDTH
@@ -564,13 +568,12 @@ class Surrogate:
         fused = self.kstep == 32
         x0 = None if fused else self.features(vel_hist, cls, pos, train=False)
         feats = (vel_hist, cls) if fused else None
-        if r is not None:   # edge sums in the graph's cell order (its `order` work array)
+        # (edge sums in the graph's cell order, forward(order=...), measured no faster: node order)
+        if r is not None:
             ci, deg, cap, eps = r
-            return self.forward(x0, pos, None, ci, N, train=False, eps=eps, rows=(deg, cap), feats=feats,
-                                order=self._graph_ws["order"])
+            return self.forward(x0, pos, None, ci, N, train=False, eps=eps, rows=(deg, cap), feats=feats)
         rp, ci, _ = self.graph(pos, scene, nscene, eps=True)
-        return self.forward(x0, pos, rp, ci, N, train=False, eps=self.graph_eps, feats=feats,
-                            order=self._graph_ws["order"])
+        return self.forward(x0, pos, rp, ci, N, train=False, eps=self.graph_eps, feats=feats)
 
     def rollout_step(self, pos, vel_hist, y, cls, body, nbodies: int, h: float, body_sums) -> None:
         """x, v history advanced by stride h from the network's output y."""
