@@ -693,13 +693,18 @@ __global__ void __launch_bounds__(256) k_edge_fwd_h(const h2_t *__restrict__ AB,
     auto edge = [&](h2_t bj, uint2 e, h2_t acc) {
         const h2_t exy = bitsh2(e.x), ezd = bitsh2(e.y);
         h2_t c = a2 + bj;
-        if constexpr (!FOLD) {
-            c = __builtin_elementwise_fma(wx, __builtin_shufflevector(exy, exy, 0, 0), c);
-            c = __builtin_elementwise_fma(wy, __builtin_shufflevector(exy, exy, 1, 1), c);
-            c = __builtin_elementwise_fma(wz, __builtin_shufflevector(ezd, ezd, 0, 0), c);
-        }
+        c = __builtin_elementwise_fma(wx, __builtin_shufflevector(exy, exy, 0, 0), c);
+        c = __builtin_elementwise_fma(wy, __builtin_shufflevector(exy, exy, 1, 1), c);
+        c = __builtin_elementwise_fma(wz, __builtin_shufflevector(ezd, ezd, 0, 0), c);
         c = __builtin_elementwise_fma(wd, __builtin_shufflevector(ezd, ezd, 1, 1), c);
         return acc + __builtin_elementwise_maximum(c, zero);   // NaN-propagating: an overflow stays visible
+    };
+    // FOLD: eps holds one half distance per entry, read as pairs (rows start at
+    // even entries and every batch but the last single edge has an even size)
+    auto edge_fold = [&](h2_t bj, h2_t dp, bool odd, h2_t acc) {
+        const h2_t c = __builtin_elementwise_fma(wd, odd ? __builtin_shufflevector(dp, dp, 1, 1)
+                                                         : __builtin_shufflevector(dp, dp, 0, 0), a2 + bj);
+        return acc + __builtin_elementwise_maximum(c, zero);
     };
     // batches of B edges: all rows requested, then the chains (which interleave)
     auto batch = [&](auto nb) {
@@ -708,8 +713,14 @@ __global__ void __launch_bounds__(256) k_edge_fwd_h(const h2_t *__restrict__ AB,
 #pragma unroll
         for (int t = 0; t < B; t++) bj[t] = b_of(ci[q + t]);
         h2_t acc = zero;
+        if constexpr (FOLD) {
+            const uint32_t *dq = reinterpret_cast<const uint32_t *>(eps) + (q >> 1);
 #pragma unroll
-        for (int t = 0; t < B; t++) acc = edge(bj[t], eps[q + t], acc);
+            for (int t = 0; t < B; t++) acc = edge_fold(bj[t], bitsh2(dq[t >> 1]), (t & 1) != 0, acc);
+        } else {
+#pragma unroll
+            for (int t = 0; t < B; t++) acc = edge(bj[t], eps[q + t], acc);
+        }
         s0 += (float)acc.x;
         s1 += (float)acc.y;
         q += B;
@@ -940,7 +951,8 @@ __global__ void __launch_bounds__(256) k_g_rows(const float *__restrict__ pos, c
                                                 const int *__restrict__ start, const float4 *__restrict__ ps, int N,
                                                 NnGrid gd, float r2, float inv_r, int *__restrict__ deg,
                                                 const int *__restrict__ rp, int *__restrict__ ci,
-                                                uint2 *__restrict__ eps, int row_cap = 0, int *over = nullptr) {
+                                                uint2 *__restrict__ eps, int row_cap = 0, int *over = nullptr,
+                                                _Float16 *__restrict__ dist = nullptr) {
     const int w = __builtin_amdgcn_readfirstlane(xcd_block(gridDim.x) * 4 + (threadIdx.x >> 6));
     if (w >= N) return;
     const int lane = threadIdx.x & 63;
@@ -976,11 +988,11 @@ __global__ void __launch_bounds__(256) k_g_rows(const float *__restrict__ pos, c
                     if (!SLAB || o < row_cap) {   // a cut row is reported, never written past
                         const size_t at = SLAB ? (size_t)i * row_cap + o : (size_t)o;
                         ci[at] = j;
-                        if (eps) {
-                            const float d = sqrtf(dx * dx + dy * dy + dz * dz);
+                        const _Float16 dh = (_Float16)(sqrtf(dx * dx + dy * dy + dz * dz) * inv_r);   // one rounding for both
+                        if (eps)
                             eps[at] = make_uint2(h2bits(h2_t{(_Float16)(dx * inv_r), (_Float16)(dy * inv_r)}),
-                                                 h2bits(h2_t{(_Float16)(dz * inv_r), (_Float16)(d * inv_r)}));
-                        }
+                                                 h2bits(h2_t{(_Float16)(dz * inv_r), dh}));
+                        if (dist) dist[at] = dh;
                     }
                 }
                 out += __popcll(m);
@@ -1491,7 +1503,7 @@ int dmph_nn_edge_fwd_f16(const uint16_t *AB, const int *rp, const int *ci, const
                          const int *order, const NnFold *fold, void *stream) {
     if (N <= 0) return 0;
     if (lds < 128 || lds % 2 || (deg && (row_cap <= 0 || (long long)N * row_cap > 0x7fffffffLL))) return -1;
-    if (fold && !fold->pos) return -1;
+    if (fold && (!fold->pos || !deg || row_cap % 2)) return -1;   // the fold reads rows' distance pairs
     const bool buf = (size_t)N * 512 <= 0xffffffffull - 1024;   // row offsets fit the 32-bit scalar offset
     const NnFold fd = fold ? *fold : NnFold{};
     auto go = [&](auto kern) {
@@ -1590,8 +1602,8 @@ int dmph_nn_graph(const float *pos, const int *scene, int N, const NnGrid *grid,
 
 int dmph_nn_graph_rows(const float *pos, const int *scene, int N, const NnGrid *grid, float radius, int *key,
                        int *counts, int *start, int *fill, int *order, int *sorted, int *deg, int *ci, uint32_t *eps,
-                       int row_cap, int *over_out, int *scratch, void *stream) {
-    if (row_cap <= 0) return -1;
+                       uint16_t *dist, int row_cap, int *over_out, int *scratch, void *stream) {
+    if (row_cap <= 0 || (dist && row_cap % 2)) return -1;
     hipStream_t s = (hipStream_t)stream;
     const int nk = grid->ncell * (grid->nscene > 0 ? grid->nscene : 1);
     int *over = scratch + 2;   // graph scratch: [2, 3) the cut-row count here
@@ -1605,7 +1617,8 @@ int dmph_nn_graph_rows(const float *pos, const int *scene, int N, const NnGrid *
     float4 *ps = reinterpret_cast<float4 *>(sorted);
     hipLaunchKernelGGL(k_g_rank, dim3(b), dim3(256), 0, s, key, start, order, pos, ps, N);
     hipLaunchKernelGGL((k_g_rows<true, true>), dim3((N + 3) / 4), dim3(256), 0, s, pos, key, start, ps, N, *grid,
-                       radius * radius, 1.f / radius, deg, nullptr, ci, reinterpret_cast<uint2 *>(eps), row_cap, over);
+                       radius * radius, 1.f / radius, deg, nullptr, ci, reinterpret_cast<uint2 *>(eps), row_cap, over,
+                       reinterpret_cast<_Float16 *>(dist));
     int cut = 0;
     if (hipMemcpyAsync(&cut, over, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess) return -5;
     if (hipStreamSynchronize(s) != hipSuccess) return -5;

@@ -158,6 +158,7 @@ class Surrogate:
         self._box = None        # host copy of the last graph's bounding box, with the positions it belongs to
         self._folded = None   # inference: per round [n1 left | n1 right @ w2] (rebuilt after a weight change)
         self._wt = None       # inference: per round Wc[:, 0:3] transposed (the position fold)
+        self.rows_dist = False  # the last graph_rows wrote distances only (for the fold)
         self.pos_fold = True  # fold the edge features' linear part into the node terms where the box allows
 
     def at_spacing(self, d: float) -> "Surrogate":
@@ -281,16 +282,21 @@ class Surrogate:
             self._graph_ws = ws
         return ws
 
-    def graph_rows(self, pos, scene=None, nscene: int = 1, row_cap: int = 160):
+    def graph_rows(self, pos, scene=None, nscene: int = 1, row_cap: int = 160, dist: Optional[bool] = None):
         """Inference graph in one pass: rows of a fixed capacity (row i at
         i * row_cap, deg[i] entries) with the edge features; the capacity grows
         while rows are cut.  Returns (ci, deg, row_cap, eps), or None where N x
-        row_cap passes int32 offsets (then use graph())."""
+        row_cap passes int32 offsets (then use graph()).  With `dist` the
+        features are the distances alone (one half per entry, what the position
+        fold reads); None: the distances where the fold applies to these nodes.
+        self.rows_dist records which was written."""
         torch = self.torch
         N = pos.shape[0]
         lib = nnlib()
         stream = C.c_void_p(torch.cuda.current_stream(self.dev).cuda_stream)
         g, nk = self._grid(pos, nscene, stream)
+        if dist is None:
+            dist = self._fold_args(pos, N) is not None
         ws = self._graph_work(N, nk)
         cap = max(ws.get("row_cap", 0), row_cap)
         over = C.c_int(0)
@@ -303,12 +309,14 @@ class Surrogate:
                 ws["row_cap"], ws["rows_n"] = cap, N
             rc = lib.dmph_nn_graph_rows(_p(pos), _p(scene), N, C.byref(g), C.c_float(self.radius), _p(ws["key"]),
                                         _p(ws["counts"]), _p(ws["start"]), _p(ws["fill"]), _p(ws["order"]),
-                                        _p(ws["sorted"]), _p(ws["deg"]), _p(ws["rows_ci"]), _p(ws["rows_eps"]), cap,
+                                        _p(ws["sorted"]), _p(ws["deg"]), _p(ws["rows_ci"]),
+                                        _p(None if dist else ws["rows_eps"]), _p(ws["rows_eps"] if dist else None), cap,
                                         C.byref(over), _p(ws["scan"]), stream)
             if rc == -2:
                 cap = (cap * 3 // 2 + 15) // 16 * 16
                 continue
             _chk(rc, "dmph_nn_graph_rows")
+            self.rows_dist = bool(dist)
             return ws["rows_ci"], ws["deg"], cap, ws["rows_eps"]
 
     # ------------------------------------------------------------------ forward / backward
@@ -441,7 +449,11 @@ class Surrogate:
         fused = not train and self.kstep == 32
         folded = None if train else self._fold()
         # the position fold needs every [A | B] from a fused kernel (encoder included)
-        fold = self._fold_args(pos, N) if fused and feats is not None else None
+        # and rows holding the distances (graph_rows with dist)
+        dist_rows = rows is not None and self.rows_dist
+        fold = self._fold_args(pos, N) if fused and feats is not None and dist_rows else None
+        if dist_rows and fold is None:
+            raise RuntimeError("the rows hold distances only: they are summed by the folded inference path")
         if fused and feats is not None:   # features, encoder and the first [A | B] in one kernel
             vel_hist, cls = feats
             _chk(lib.dmph_nn_encode(_p(vel_hist), N, _p(cls), _p(pos), C.byref(self._feat()), _p(self.enc1.wb),

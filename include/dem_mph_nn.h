@@ -111,9 +111,10 @@ int dmph_nn_edge_fwd(const uint16_t *AB, const float *pos, const int *rp, const 
  * edges added into FP32 sums; S as in edge_fwd (bf16).  With deg (not NULL) the
  * rows are those of dmph_nn_graph_rows: row i at i * row_cap, deg[i] entries
  * (rp unused), edge_eps not needed (the graph writes eps).  With fold (not
- * NULL) the B half of AB holds + Wc3 P_j (written by a folding dmph_nn_node or
- * dmph_nn_encode with the same origin and inv_r), A_i gets - Wc3 P_i here and
- * only the distance of eps is read: one packed multiply-add per edge instead of four.
+ * NULL; rows only) the B half of AB holds + Wc3 P_j (written by a folding
+ * dmph_nn_node or dmph_nn_encode with the same origin and inv_r), A_i gets
+ * - Wc3 P_i here, and eps is the `dist` array of dmph_nn_graph_rows (one half
+ * per entry): one packed multiply-add per edge instead of four.
  * order (NULL or a permutation of [0, N), e.g. the `order` work array of the
  * graph builders, which lists the nodes by cell): the nodes are summed in that
  * order, so the rows of B read at one time are neighbours in space and come
@@ -143,12 +144,14 @@ int dmph_nn_graph(const float *pos, const int *scene, int N, const NnGrid *grid,
 
 /* the same graph in rows of a fixed capacity, one pass (no degree count first):
  * row i at ci + i * row_cap with deg[i] entries, same order as dmph_nn_graph;
- * eps (NULL or [2 N row_cap] words) as edge_eps writes it.  Returns -2 with
- * *over_out = rows longer than row_cap (their tails unwritten: call again with a
- * larger row_cap or use dmph_nn_graph); same work arrays and scratch. */
+ * eps (NULL or [2 N row_cap] words) as edge_eps writes it; dist (NULL or
+ * [N row_cap] IEEE halves) the distance alone, |x_j - x_i| / radius, for the
+ * folded edge sums (row_cap even).  Returns -2 with *over_out = rows longer than
+ * row_cap (their tails unwritten: call again with a larger row_cap or use
+ * dmph_nn_graph); same work arrays and scratch. */
 int dmph_nn_graph_rows(const float *pos, const int *scene, int N, const NnGrid *grid, float radius, int *key,
                        int *counts, int *start, int *fill, int *order, int *sorted, int *deg, int *ci, uint32_t *eps,
-                       int row_cap, int *over_out, int *scratch, void *stream);
+                       uint16_t *dist, int row_cap, int *over_out, int *scratch, void *stream);
 
 /* bounding box of pos [N][3] into out[6] (min x y z, max x y z; device memory);
  * scratch of dmph_nn_bbox_scratch(N) bytes */

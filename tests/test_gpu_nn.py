@@ -247,7 +247,7 @@ def test_fixed_capacity_rows_equal_the_csr_graph(S):
     rp, ci, nnz = sur.graph(pos, eps=True)
     RP, CI = rp.cpu().numpy(), ci.cpu().numpy()
     E = sur.graph_eps[:2 * nnz].cpu().numpy().reshape(-1, 2)
-    ci2, deg, cap, eps2 = sur.graph_rows(pos, row_cap=16)
+    ci2, deg, cap, eps2 = sur.graph_rows(pos, row_cap=16, dist=False)
     assert cap >= np.diff(RP).max() and cap % 16 == 0
     D, C2, E2 = deg[:n].cpu().numpy(), ci2.cpu().numpy().reshape(n, cap), eps2.cpu().numpy().reshape(n, cap, 2)
     np.testing.assert_array_equal(D, np.diff(RP))
@@ -274,6 +274,14 @@ def test_fixed_capacity_rows_equal_the_csr_graph(S):
     assert lib.dmph_nn_edge_fwd_f16(_p(abd), None, _p(ci2), _p(eps2), _p(Wc), _p(bc), _p(o3), 128, n, _p(deg), cap,
                                     _p(order), None, s) == 0
     assert torch.equal(o1, o3)
+    # distances alone (for the position fold): the distance halves of eps, same rows
+    ci3, deg3, cap3, d3 = sur.graph_rows(pos, row_cap=16, dist=True)
+    assert sur.rows_dist and cap3 == cap and torch.equal(deg3[:n], deg[:n])
+    D3 = d3.cpu().numpy().view(np.uint16)[:n * cap].reshape(n, cap)
+    C3 = ci3.cpu().numpy().reshape(n, cap)
+    for i in range(n):
+        np.testing.assert_array_equal(C3[i, :D[i]], CI[RP[i]:RP[i + 1]])
+        np.testing.assert_array_equal(D3[i, :D[i]], (E[RP[i]:RP[i + 1], 1].astype(np.uint32) >> 16).astype(np.uint16))
 
 
 def test_radius_graph_refuses_more_than_int32_edges(S):
@@ -393,19 +401,17 @@ def _fold_arg(S, pos_t, origin, inv_r, Wt=None):
 @pytest.mark.parametrize("reach", [0.0, 12.0])
 def test_folded_half_edge_sums_match_fp64(S, reach):
     """Position fold: with B_j + Wc3 P_j (P = (x - origin) / r, rounded to half
-    on the host) and the fold passed, the edge kernel subtracts Wc3 P_i from A_i
-    and multiplies only the distance per edge.  The sums equal the FP64 sums of
-    the unfolded formula to the unfolded kernel's tolerance, also with the
-    origin moved so that |P| reaches 16 radii (the driver's limit); rows of
-    every length as in the unfolded test."""
+    on the host), the rows' distances and the fold passed, the edge kernel
+    subtracts Wc3 P_i from A_i and multiplies only the distance per edge.  The
+    sums equal the FP64 sums of the unfolded formula to the unfolded kernel's
+    tolerance, also with the origin moved so that |P| reaches 16 radii (the
+    driver's limit); rows cut to i % 71 entries (empty, tails of every length)."""
     sur, sc, pos, vh, cls, tgt, mask = _small_problem(S, seed=3)
     n = sc.n
-    rp0, ci0, nnz0 = sur.graph(pos)
-    RP0, CI0 = rp0.cpu().numpy(), ci0.cpu().numpy()
-    lens = np.minimum(np.diff(RP0), np.arange(n) % 71)
-    RP = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
-    CI = np.concatenate([CI0[RP0[i]:RP0[i] + lens[i]] for i in range(n)]).astype(np.int32)
-    rp, ci = torch.tensor(RP, device="cuda:0"), torch.tensor(CI, device="cuda:0")
+    ci, deg, cap, dist = sur.graph_rows(pos, row_cap=16, dist=True)
+    D = np.minimum(deg[:n].cpu().numpy(), np.arange(n) % 71).astype(np.int32)
+    degc = torch.tensor(D, device="cuda:0")
+    CI = ci.cpu().numpy().reshape(-1)[:n * cap].reshape(n, cap)
     rng = np.random.default_rng(3)
     ab = (rng.standard_normal((n, 256)) * 0.7).astype(np.float16)
     Wc = (rng.standard_normal((128, 4)) * 0.8).astype(np.float32)
@@ -421,17 +427,19 @@ def test_folded_half_edge_sums_match_fp64(S, reach):
     s = C.c_void_p(torch.cuda.current_stream().cuda_stream)
     abd = torch.tensor(abf.view(np.int16), device="cuda:0")
     Wcd, bcd = torch.tensor(Wc, device="cuda:0"), torch.tensor(bc, device="cuda:0")
-    eps = torch.empty(2 * ci.numel(), dtype=torch.int32, device="cuda:0")
     out = torch.zeros((n, 128), dtype=torch.int16, device="cuda:0")
     fd = _fold_arg(S, pos, origin, 1.0 / r)
-    assert lib.dmph_nn_edge_eps(_p(pos), _p(rp), _p(ci), _p(eps), n, C.c_float(r), s) == 0
-    assert lib.dmph_nn_edge_fwd_f16(_p(abd), _p(rp), _p(ci), _p(eps), _p(Wcd), _p(bcd), _p(out), 128, n,
-                                    None, 0, None, C.byref(fd), s) == 0
+    assert lib.dmph_nn_edge_fwd_f16(_p(abd), None, _p(ci), _p(dist), _p(Wcd), _p(bcd), _p(out), 128, n,
+                                    _p(degc), cap, None, C.byref(fd), s) == 0
+    # the fold needs rows (distance pairs): CSR is refused
+    assert lib.dmph_nn_edge_fwd_f16(_p(abd), _p(degc), _p(ci), _p(dist), _p(Wcd), _p(bcd), _p(out), 128, n,
+                                    None, 0, None, C.byref(fd), s) == -1
     got = _bf16_float(out.cpu().numpy().view(np.uint16))
     A, B = ab[:, :128].astype(np.float64), ab[:, 128:].astype(np.float64)
     ref = np.zeros((n, 128))
+    assert D.min() == 0 and D.max() == 70
     for i in range(n):
-        J = CI[RP[i]:RP[i + 1]]
+        J = CI[i, :D[i]]
         d = P[J] - P[i]
         e = np.concatenate([d, np.linalg.norm(d, axis=1, keepdims=True)], axis=1) / r
         ref[i] = np.maximum(A[i] + bc + B[J] + e @ Wc.T.astype(np.float64), 0.0).sum(axis=0)
