@@ -29,12 +29,22 @@ namespace nn {
 typedef uint16_t bf16_t;   // storage
 
 __device__ __forceinline__ float bf2f(bf16_t v) { return __uint_as_float(((uint32_t)v) << 16); }
-// round to nearest even
-__device__ __forceinline__ bf16_t f2bf(float f) {
-    uint32_t u = __float_as_uint(f);
-    if ((u & 0x7f800000u) == 0x7f800000u) return (bf16_t)(u >> 16 | ((u & 0xffffu) ? 0x40u : 0u));   // inf / nan
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return (bf16_t)(u >> 16);
+// round to nearest even (NaN stays NaN): gfx950's v_cvt_pk_bf16_f32, two values per instruction
+typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+typedef _Float16 h2_t __attribute__((ext_vector_type(2)));
+typedef float f32x2_t __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint32_t pk_bf16(float lo, float hi) {
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector(f32x2_t{lo, hi}, bf16x2_t));
+}
+__device__ __forceinline__ bf16_t f2bf(float f) { return __builtin_bit_cast(bf16_t, (__bf16)f); }
+// the same rounding in integer arithmetic (one value at a time: fewer live registers)
+__device__ __forceinline__ uint32_t pk_bf16_int(float lo, float hi) {
+    auto one = [](float f) {
+        uint32_t u = __float_as_uint(f);
+        if ((u & 0x7f800000u) == 0x7f800000u) return u >> 16 | ((u & 0xffffu) ? 0x40u : 0u);   // inf / nan
+        return (u + 0x7fffu + ((u >> 16) & 1u)) >> 16;
+    };
+    return one(lo) | one(hi) << 16;
 }
 
 typedef short s4_t __attribute__((ext_vector_type(4)));
@@ -214,11 +224,15 @@ __global__ void __launch_bounds__(NW * 64) k_gemm(const bf16_t *__restrict__ X, 
                     if (relu)
 #pragma unroll
                         for (int e = 0; e < 4; e++) v[e] = fmaxf(v[e], 0.f);
-                    uint32_t o[4];
-#pragma unroll
-                    for (int e = 0; e < 4; e++)
-                        o[e] = half ? (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)v[e]) : (uint32_t)f2bf(v[e]);
-                    *reinterpret_cast<uint2 *>(Y + (size_t)r * ldy + col) = make_uint2(o[0] | o[1] << 16, o[2] | o[3] << 16);
+                    uint2 o;
+                    if (half) {
+                        o.x = __builtin_bit_cast(uint32_t, h2_t{(_Float16)v[0], (_Float16)v[1]});
+                        o.y = __builtin_bit_cast(uint32_t, h2_t{(_Float16)v[2], (_Float16)v[3]});
+                    } else {
+                        o.x = pk_bf16(v[0], v[1]);
+                        o.y = pk_bf16(v[2], v[3]);
+                    }
+                    *reinterpret_cast<uint2 *>(Y + (size_t)r * ldy + col) = o;
                 }
             }
         }
@@ -326,8 +340,8 @@ __global__ void __launch_bounds__(NWV * 64) k_node(const bf16_t *__restrict__ HS
             const float4 b = *reinterpret_cast<const float4 *>(b1 + c * 16 + 4 * g);
             const float v0 = fmaxf(acc[c][0] + b.x, 0.f), v1 = fmaxf(acc[c][1] + b.y, 0.f);
             const float v2 = fmaxf(acc[c][2] + b.z, 0.f), v3 = fmaxf(acc[c][3] + b.w, 0.f);
-            uf[c][0] = (uint32_t)f2bf(v0) | (uint32_t)f2bf(v1) << 16;
-            uf[c][1] = (uint32_t)f2bf(v2) | (uint32_t)f2bf(v3) << 16;
+            uf[c][0] = pk_bf16_int(v0, v1);
+            uf[c][1] = pk_bf16_int(v2, v3);
         }
         // the residual h in the output layout, requested before the second product
         uint2 hr[8];
@@ -359,8 +373,8 @@ __global__ void __launch_bounds__(NWV * 64) k_node(const bf16_t *__restrict__ HS
             const float v1 = acc[c][1] + b.y + bf2f((bf16_t)(hr[c].x >> 16));
             const float v2 = acc[c][2] + b.z + bf2f((bf16_t)(hr[c].y & 0xffff));
             const float v3 = acc[c][3] + b.w + bf2f((bf16_t)(hr[c].y >> 16));
-            hf[c][0] = (uint32_t)f2bf(v0) | (uint32_t)f2bf(v1) << 16;
-            hf[c][1] = (uint32_t)f2bf(v2) | (uint32_t)f2bf(v3) << 16;
+            hf[c][0] = pk_bf16(v0, v1);
+            hf[c][1] = pk_bf16(v2, v3);
             if (!DEC && r < M)
                 *reinterpret_cast<uint2 *>(Hn + (size_t)r * ldhn + c * 16 + 4 * g) = make_uint2(hf[c][0], hf[c][1]);
         }
@@ -611,10 +625,9 @@ __global__ void __launch_bounds__(256) k_edge_fwd(const bf16_t *__restrict__ AB,
             }
         }
     }
-    *reinterpret_cast<uint32_t *>(S + (size_t)i * lds + f) = (uint32_t)f2bf(s0) | ((uint32_t)f2bf(s1) << 16);
+    *reinterpret_cast<uint32_t *>(S + (size_t)i * lds + f) = pk_bf16(s0, s1);
 }
 
-typedef _Float16 h2_t __attribute__((ext_vector_type(2)));
 
 __device__ __forceinline__ uint32_t h2bits(h2_t v) { return __builtin_bit_cast(uint32_t, v); }
 __device__ __forceinline__ h2_t bitsh2(uint32_t v) { return __builtin_bit_cast(h2_t, v); }
@@ -732,7 +745,7 @@ __global__ void __launch_bounds__(256) k_edge_fwd_h(const h2_t *__restrict__ AB,
     if (q + 4 <= q1) batch(std::integral_constant<int, 4>{});
     if (q + 2 <= q1) batch(std::integral_constant<int, 2>{});
     if (q < q1) batch(std::integral_constant<int, 1>{});
-    *reinterpret_cast<uint32_t *>(S + (size_t)i * lds + f) = (uint32_t)f2bf(s0) | ((uint32_t)f2bf(s1) << 16);
+    *reinterpret_cast<uint32_t *>(S + (size_t)i * lds + f) = pk_bf16(s0, s1);
 }
 
 // dS: [N][128] FP32 (gradient of S).  Outputs dAB [N][256] bf16 and per-block
@@ -775,8 +788,8 @@ __global__ void __launch_bounds__(256) k_edge_bwd(const bf16_t *__restrict__ AB,
             db0 += d0 > 0.f ? dsj.x : 0.f;
             db1 += d1 > 0.f ? dsj.y : 0.f;
         }
-        *reinterpret_cast<uint32_t *>(dAB + (size_t)n * 256 + f) = (uint32_t)f2bf(da0) | ((uint32_t)f2bf(da1) << 16);
-        *reinterpret_cast<uint32_t *>(dAB + (size_t)n * 256 + 128 + f) = (uint32_t)f2bf(db0) | ((uint32_t)f2bf(db1) << 16);
+        *reinterpret_cast<uint32_t *>(dAB + (size_t)n * 256 + f) = pk_bf16(da0, da1);
+        *reinterpret_cast<uint32_t *>(dAB + (size_t)n * 256 + 128 + f) = pk_bf16(db0, db1);
     }
     for (int t = 0; t < 4; t++) { red[wave][4 * f + t] = gw[t]; red[wave][4 * (f + 1) + t] = gw[4 + t]; }
     red[wave][512 + f] = gw[8];
@@ -1080,7 +1093,7 @@ __global__ void k_head_bwd(const bf16_t *H, const float *Wd, const float *dY, bf
         float d1 = y[0] * Wd[f + 1] + y[1] * Wd[128 + f + 1] + y[2] * Wd[256 + f + 1];
         if (!(h0 > 0.f)) d0 = 0.f;
         if (!(h1 > 0.f)) d1 = 0.f;
-        *reinterpret_cast<uint32_t *>(dH + (size_t)i * 128 + f) = (uint32_t)f2bf(d0) | ((uint32_t)f2bf(d1) << 16);
+        *reinterpret_cast<uint32_t *>(dH + (size_t)i * 128 + f) = pk_bf16(d0, d1);
         for (int c = 0; c < 3; c++) { g[2 * c] = y[c] * h0; g[2 * c + 1] = y[c] * h1; yb[c] = y[c]; }
     }
     for (int c = 0; c < 3; c++) { red[wave][c * 128 + f] = g[2 * c]; red[wave][c * 128 + f + 1] = g[2 * c + 1]; }
@@ -1318,8 +1331,8 @@ __global__ void __launch_bounds__(kNodeWaves * 64) k_enc(const float *__restrict
 #pragma unroll
         for (int c = 0; c < 8; c++) {   // h1 = relu(We1 x + be1)
             const float4 b = *reinterpret_cast<const float4 *>(be1 + c * 16 + 4 * g);
-            hf[c][0] = (uint32_t)f2bf(fmaxf(acc[c][0] + b.x, 0.f)) | (uint32_t)f2bf(fmaxf(acc[c][1] + b.y, 0.f)) << 16;
-            hf[c][1] = (uint32_t)f2bf(fmaxf(acc[c][2] + b.z, 0.f)) | (uint32_t)f2bf(fmaxf(acc[c][3] + b.w, 0.f)) << 16;
+            hf[c][0] = pk_bf16_int(fmaxf(acc[c][0] + b.x, 0.f), fmaxf(acc[c][1] + b.y, 0.f));
+            hf[c][1] = pk_bf16_int(fmaxf(acc[c][2] + b.z, 0.f), fmaxf(acc[c][3] + b.w, 0.f));
         }
 #pragma unroll
         for (int c = 0; c < 8; c++) acc[c] = f4_t{0.f, 0.f, 0.f, 0.f};
@@ -1336,8 +1349,8 @@ __global__ void __launch_bounds__(kNodeWaves * 64) k_enc(const float *__restrict
 #pragma unroll
         for (int c = 0; c < 8; c++) {
             const float4 b = *reinterpret_cast<const float4 *>(be2 + c * 16 + 4 * g);
-            hf[c][0] = (uint32_t)f2bf(acc[c][0] + b.x) | (uint32_t)f2bf(acc[c][1] + b.y) << 16;
-            hf[c][1] = (uint32_t)f2bf(acc[c][2] + b.z) | (uint32_t)f2bf(acc[c][3] + b.w) << 16;
+            hf[c][0] = pk_bf16(acc[c][0] + b.x, acc[c][1] + b.y);
+            hf[c][1] = pk_bf16(acc[c][2] + b.z, acc[c][3] + b.w);
             if (r < M) *reinterpret_cast<uint2 *>(H0 + (size_t)r * ldh0 + c * 16 + 4 * g) = make_uint2(hf[c][0], hf[c][1]);
         }
         float px = 0.f, py = 0.f, pz = 0.f;
