@@ -293,7 +293,9 @@ def surrogate_bench(D, scenes, seq_ms_per_dt: float, batched_ms_per_dt: float, c
     """The learned surrogate on the ensemble: throughput of surrogate + fallback
     (one plain-solver stride to re-anchor every 200 surrogate steps, charged at
     the batched solver's rate) on all scenes in chunks, and accuracy on held-out
-    scenes against the plain solver next to the two baselines without a network."""
+    scenes against the FP64 solver (the GPU parity mode, equal to the FP64
+    oracle to ~1e-9 per step) next to the two baselines without a network and
+    the mixed-precision solver's own error."""
     from dem_mph_amd import scenes as S
     from dem_mph_amd import surrogate as SG
     if not os.path.exists(SURROGATE_WEIGHTS):
@@ -320,7 +322,7 @@ def surrogate_bench(D, scenes, seq_ms_per_dt: float, batched_ms_per_dt: float, c
            "vs_sequential_plain": (SG.STRIDE * dt / t_total) / (dt / (seq_ms_per_dt * 1e-3)),
            "vs_batched_plain": (SG.STRIDE * dt / t_total) / (dt / (batched_ms_per_dt * 1e-3))}
     ev = S.ensemble_c3(eval_scenes, seed=seed + 1000)
-    out["accuracy"] = SG.evaluate(sur, ev, params, 2000, eval_steps, reanchor, 0.008)
+    out["accuracy"] = SG.evaluate(sur, ev, params, 2000, eval_steps, reanchor, 0.008, fp64=True)
     return out
 
 

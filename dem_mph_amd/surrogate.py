@@ -608,13 +608,15 @@ class Trajectories:
     `samples` lists the frames usable as a training sample (the 4 frames before
     it and the one after it are `stride` steps apart)."""
 
-    def __init__(self, scenes, params, steps: List[int], stride: int = STRIDE, device: str = "cuda:0"):
+    def __init__(self, scenes, params, steps: List[int], stride: int = STRIDE, device: str = "cuda:0",
+                 precision: Optional[int] = None):
         import torch
         import dem_mph_amd as D
         self.torch = torch
         dev = torch.device(device)
         t0 = _now()
-        s = D.Solver(D.Params.from_dict(params), scenes, precision=D.PREC_FP32, rigid=True)
+        self.precision = D.PREC_FP32 if precision is None else precision   # FP64: the parity solver
+        s = D.Solver(D.Params.from_dict(params), scenes, precision=self.precision, rigid=True)
         s.autotune(3)
         self.off = np.asarray(s.scene_offsets)
         self.dt = params["Dt"]
@@ -841,12 +843,44 @@ def load(path: str, kstep: int = 32) -> Surrogate:
     return sur
 
 
-def evaluate(sur: Surrogate, scenes, params, start: int, steps: int, reanchor: int, tol: float) -> Dict[str, object]:
-    """Held-out scenes: reference frames from the plain solver, then the
-    surrogate's rollout and the two baselines without a network."""
-    traj = Trajectories(scenes, params, [start - (HIST - 1) * STRIDE + k * STRIDE for k in range(steps + HIST)])
+def frame_errors(ref: "Trajectories", other: "Trajectories", frames: List[int]) -> Dict[str, float]:
+    """Per-scene debris-centroid and fluid RMS of `other` against `ref` (same
+    scenes and solver steps), the largest over `frames`, then mean / max over
+    the scenes."""
+    ed = np.zeros(ref.nscene)
+    ef = np.zeros(ref.nscene)
+    for f in frames:
+        a = ref.pos[f].cpu().numpy().astype(np.float64)
+        b = other.pos[f].cpu().numpy().astype(np.float64)
+        ca, cb = ref.centroids(a), ref.centroids(b)
+        for q in range(ref.nscene):
+            sl = slice(int(ref.off[q]), int(ref.off[q + 1]))
+            bq = np.unique(ref.body_np[sl][ref.body_np[sl] >= 0])
+            if len(bq):
+                ed[q] = max(ed[q], float(np.sqrt(((cb[bq] - ca[bq]) ** 2).sum(axis=1).mean())))
+            fl = ref.cls_np[sl] == 0
+            if fl.any():
+                ef[q] = max(ef[q], float(np.sqrt(((b[sl][fl] - a[sl][fl]) ** 2).sum(axis=1).mean())))
+    return {"debris_centroid_rms_m_mean": float(ed.mean()), "debris_centroid_rms_m_max": float(ed.max()),
+            "fluid_rms_m_mean": float(ef.mean())}
+
+
+def evaluate(sur: Surrogate, scenes, params, start: int, steps: int, reanchor: int, tol: float,
+             fp64: bool = False) -> Dict[str, object]:
+    """Held-out scenes: reference frames from the mixed-precision solver (or,
+    with `fp64`, from the FP64 parity solver, and then also the mixed solver's
+    own error against it over the same frames), then the surrogate's rollout and
+    the two baselines without a network."""
+    import dem_mph_amd as D
+    frames = [start - (HIST - 1) * STRIDE + k * STRIDE for k in range(steps + HIST)]
+    traj = Trajectories(scenes, params, frames, precision=D.PREC_FP64 if fp64 else None)
     out = {"scenes": len(scenes), "particles": int(traj.off[-1]), "start_time_s": start * traj.dt,
-           "horizon_s": steps * STRIDE * traj.dt, "reanchor_steps": reanchor, "tolerance_m": tol}
+           "horizon_s": steps * STRIDE * traj.dt, "reanchor_steps": reanchor, "tolerance_m": tol,
+           "reference": "FP64 solver" if fp64 else "mixed-precision solver"}
+    if fp64:
+        mixed = Trajectories(scenes, params, frames)
+        out["mixed_solver"] = frame_errors(traj, mixed, list(range(HIST - 1, len(frames))))
+        del mixed
     for name, base in (("surrogate", None), ("still", "still"), ("gravity", "gravity")):
         r = rollout(sur if base is None else None, traj, HIST - 1, steps, reanchor, tol, baseline=base)
         out[name] = {"debris_centroid_rms_m_mean": float(r["debris_rms_max"].mean()),
