@@ -256,20 +256,29 @@ static unsigned gemm_grid_x(int M, int rows, bool persistent) {
 //   AB = Wab h'                    Wab [256][128], stored IEEE half (the next round's [A | B])
 // All three weights sit in LDS (160 KB, one workgroup of 16 waves per CU, each
 // wave 16 rows of a 256-row tile, tiles walked persistently).  An MFMA result
-// tile leaves lane (r, g) with 4 adjacent outputs 16c + 4g + [0, 4) of row r, so
-// the 8 values a lane feeds the next product at k step kk are outputs
-// 32kk + 4g + [0, 4) and 32kk + 16 + 4g + [0, 4): W2 and Wab are staged with
-// their columns in that order (pk_col), which makes the product the same sum.
+// tile c leaves lane (r, g) with the outputs of staged weight rows 16c + 4g +
+// [0, 4) for row r.  The weights are staged with their rows permuted (rho) so
+// that these are features 32 (c / 2) + 8g + 4 (c % 2) + [0, 4): tiles 2s and
+// 2s + 1 give lane g the 8 adjacent features 32s + 8g + [0, 8).  That is the
+// k order the next product's B operand takes at k step s (no column
+// permutation), and every store or residual load is 16 bytes of a row per lane,
+// 64 contiguous bytes per row over its four lanes.
 // ---------------------------------------------------------------------------------------------
 
 constexpr int kNodeWaves = 16;
 constexpr size_t kNodeLds = (size_t)(128 * 256 + 128 * 128 + 256 * 128) * sizeof(bf16_t);   // 160 KB
 
 // source column of staged column k (within its 32-column step) for a product fed by MFMA outputs
+// of unpermuted weights (the encoder)
 __device__ __forceinline__ int pk_col(int k) {
     const int base = k & ~31, g = (k >> 3) & 3, t = k & 7;
     return base + (t < 4 ? 4 * g + t : 16 + 4 * g + t - 4);
 }
+
+// output feature of staged weight row 16c + m (row permutation of the node kernels)
+__device__ __forceinline__ int rho(int c, int m) { return 32 * (c >> 1) + 8 * (m >> 2) + 4 * (c & 1) + (m & 3); }
+// first of the 4 features lane group g holds from tile c
+__device__ __forceinline__ int rho_base(int c, int g) { return 32 * (c >> 1) + 8 * g + 4 * (c & 1); }
 
 // DEC (the last round): the third product is the decoder's hidden layer
 // instead, g = relu(Wab h' + bab) with Wab [128][128], and its bf16-rounded
@@ -304,18 +313,18 @@ __global__ void __launch_bounds__(NWV * 64) k_node(const bf16_t *__restrict__ HS
         }
     };
     load_x((tile * NWV + wave) * 16);
-    for (int t = threadIdx.x; t < 128 * 256 / 8; t += blockDim.x) {   // W1 as is
+    for (int t = threadIdx.x; t < 128 * 256 / 8; t += blockDim.x) {   // W1, rows in rho order
         const int r = t / 32, c = (t % 32) * 8;
-        *reinterpret_cast<uint4 *>(w1 + wl_at(r, c, 256)) = *reinterpret_cast<const uint4 *>(W1 + (size_t)r * 256 + c);
+        *reinterpret_cast<uint4 *>(w1 + wl_at(r, c, 256)) =
+            *reinterpret_cast<const uint4 *>(W1 + (size_t)rho(r >> 4, r & 15) * 256 + c);
     }
-    for (int t = threadIdx.x; t < (128 + NAB) * 128 / 8; t += blockDim.x) {   // W2 and Wab, columns in pk_col order
+    for (int t = threadIdx.x; t < (128 + NAB) * 128 / 8; t += blockDim.x) {   // W2 and Wab, rows in rho order
         const int r = t / 16, c = (t % 16) * 8;
-        const bf16_t *src = r < 128 ? W2 + (size_t)r * 128 : Wab + (size_t)(r - 128) * 128;
-        bf16_t *dst = r < 128 ? w2 : wab;
-        const int rr = r < 128 ? r : r - 128;
-        const uint2 lo = *reinterpret_cast<const uint2 *>(src + pk_col(c));
-        const uint2 hi = *reinterpret_cast<const uint2 *>(src + pk_col(c + 4));
-        *reinterpret_cast<uint4 *>(dst + wl_at(rr, c, 128)) = make_uint4(lo.x, lo.y, hi.x, hi.y);
+        const int rr = r < 128 ? r : r - 128, q = rr & 127;   // Wab: rho within each half of 128 rows
+        const int src_row = (rr & ~127) + rho(q >> 4, q & 15);
+        const bf16_t *src = (r < 128 ? W2 : Wab) + (size_t)src_row * 128;
+        *reinterpret_cast<uint4 *>((r < 128 ? w2 : wab) + wl_at(rr, c, 128)) =
+            *reinterpret_cast<const uint4 *>(src + c);
     }
     __syncthreads();
     while (true) {
@@ -337,18 +346,20 @@ __global__ void __launch_bounds__(NWV * 64) k_node(const bf16_t *__restrict__ HS
         uint32_t uf[8][2];   // tile c of u, 4 bf16 = 2 words
 #pragma unroll
         for (int c = 0; c < 8; c++) {
-            const float4 b = *reinterpret_cast<const float4 *>(b1 + c * 16 + 4 * g);
+            const float4 b = *reinterpret_cast<const float4 *>(b1 + rho_base(c, g));
             const float v0 = fmaxf(acc[c][0] + b.x, 0.f), v1 = fmaxf(acc[c][1] + b.y, 0.f);
             const float v2 = fmaxf(acc[c][2] + b.z, 0.f), v3 = fmaxf(acc[c][3] + b.w, 0.f);
             uf[c][0] = pk_bf16_int(v0, v1);
             uf[c][1] = pk_bf16_int(v2, v3);
         }
-        // the residual h in the output layout, requested before the second product
+        // the residual h in the output layout (16 bytes per tile pair), requested before the second product
         uint2 hr[8];
 #pragma unroll
-        for (int c = 0; c < 8; c++) {
-            hr[c] = make_uint2(0, 0);
-            if (r < M) hr[c] = *reinterpret_cast<const uint2 *>(HS + (size_t)r * ldhs + c * 16 + 4 * g);
+        for (int s2 = 0; s2 < 4; s2++) {
+            uint4 v = make_uint4(0, 0, 0, 0);
+            if (r < M) v = *reinterpret_cast<const uint4 *>(HS + (size_t)r * ldhs + 32 * s2 + 8 * g);
+            hr[2 * s2] = make_uint2(v.x, v.y);
+            hr[2 * s2 + 1] = make_uint2(v.z, v.w);
         }
         // h' = h + W2 u + b2
 #pragma unroll
@@ -368,15 +379,16 @@ __global__ void __launch_bounds__(NWV * 64) k_node(const bf16_t *__restrict__ HS
         uint32_t hf[8][2];
 #pragma unroll
         for (int c = 0; c < 8; c++) {
-            const float4 b = *reinterpret_cast<const float4 *>(b2 + c * 16 + 4 * g);
+            const float4 b = *reinterpret_cast<const float4 *>(b2 + rho_base(c, g));
             const float v0 = acc[c][0] + b.x + bf2f((bf16_t)(hr[c].x & 0xffff));
             const float v1 = acc[c][1] + b.y + bf2f((bf16_t)(hr[c].x >> 16));
             const float v2 = acc[c][2] + b.z + bf2f((bf16_t)(hr[c].y & 0xffff));
             const float v3 = acc[c][3] + b.w + bf2f((bf16_t)(hr[c].y >> 16));
             hf[c][0] = pk_bf16(v0, v1);
             hf[c][1] = pk_bf16(v2, v3);
-            if (!DEC && r < M)
-                *reinterpret_cast<uint2 *>(Hn + (size_t)r * ldhn + c * 16 + 4 * g) = make_uint2(hf[c][0], hf[c][1]);
+            if (!DEC && (c & 1) && r < M)   // tiles c - 1 and c: 8 adjacent features
+                *reinterpret_cast<uint4 *>(Hn + (size_t)r * ldhn + rho_base(c - 1, g)) =
+                    make_uint4(hf[c - 1][0], hf[c - 1][1], hf[c][0], hf[c][1]);
         }
         if constexpr (DEC) {   // g = relu(Wab h' + bab), then the 3-output head
 #pragma unroll
@@ -395,7 +407,7 @@ __global__ void __launch_bounds__(NWV * 64) k_node(const bf16_t *__restrict__ HS
 #pragma unroll
             for (int c = 0; c < 8; c++) {
                 // (lb - lr) = 0, opaque: the head's weights are re-read per tile, not held in registers
-                const int col = c * 16 + 4 * g + (lb - lr);
+                const int col = rho_base(c, g) + (lb - lr);
                 const float4 b = *reinterpret_cast<const float4 *>(bab + col);
                 const float bb[4] = {b.x, b.y, b.z, b.w};
 #pragma unroll
@@ -416,6 +428,7 @@ __global__ void __launch_bounds__(NWV * 64) k_node(const bf16_t *__restrict__ HS
                 for (int o = 0; o < 3; o++) Y3[3 * (size_t)r + o] = part[o] + bd2[o];
         }
         // AB = Wab h', in two halves of 128 outputs
+        uint2 ab2[8];
         float px = 0.f, py = 0.f, pz = 0.f;
         if (FOLD && r < M) {
             px = (fold.pos[3 * (size_t)r] - fold.origin[0]) * fold.inv_r;
@@ -441,7 +454,7 @@ __global__ void __launch_bounds__(NWV * 64) k_node(const bf16_t *__restrict__ HS
 #pragma unroll
                 for (int c = 0; c < 8; c++) {
                     if (FOLD && half == 1) {   // + Wc3 P of the row
-                        const int col = c * 16 + 4 * g + (lb - lr);
+                        const int col = rho_base(c, g) + (lb - lr);
                         const float4 wx = *reinterpret_cast<const float4 *>(fold.Wt + col);
                         const float4 wy = *reinterpret_cast<const float4 *>(fold.Wt + 128 + col);
                         const float4 wz = *reinterpret_cast<const float4 *>(fold.Wt + 256 + col);
@@ -450,11 +463,11 @@ __global__ void __launch_bounds__(NWV * 64) k_node(const bf16_t *__restrict__ HS
                         acc[c][2] += wx.z * px + wy.z * py + wz.z * pz;
                         acc[c][3] += wx.w * px + wy.w * py + wz.w * pz;
                     }
-                    uint32_t o[4];
-#pragma unroll
-                    for (int e = 0; e < 4; e++) o[e] = (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)acc[c][e]);
-                    *reinterpret_cast<uint2 *>(ABo + (size_t)r * 256 + half * 128 + c * 16 + 4 * g) =
-                        make_uint2(o[0] | o[1] << 16, o[2] | o[3] << 16);
+                    ab2[c] = make_uint2(__builtin_bit_cast(uint32_t, h2_t{(_Float16)acc[c][0], (_Float16)acc[c][1]}),
+                                        __builtin_bit_cast(uint32_t, h2_t{(_Float16)acc[c][2], (_Float16)acc[c][3]}));
+                    if (c & 1)   // tiles c - 1 and c: 8 adjacent outputs
+                        *reinterpret_cast<uint4 *>(ABo + (size_t)r * 256 + half * 128 + rho_base(c - 1, g)) =
+                            make_uint4(ab2[c - 1].x, ab2[c - 1].y, ab2[c].x, ab2[c].y);
                 }
         }
         if (next >= ntiles) break;
