@@ -268,13 +268,6 @@ static unsigned gemm_grid_x(int M, int rows, bool persistent) {
 constexpr int kNodeWaves = 16;
 constexpr size_t kNodeLds = (size_t)(128 * 256 + 128 * 128 + 256 * 128) * sizeof(bf16_t);   // 160 KB
 
-// source column of staged column k (within its 32-column step) for a product fed by MFMA outputs
-// of unpermuted weights (the encoder)
-__device__ __forceinline__ int pk_col(int k) {
-    const int base = k & ~31, g = (k >> 3) & 3, t = k & 7;
-    return base + (t < 4 ? 4 * g + t : 16 + 4 * g + t - 4);
-}
-
 // output feature of staged weight row 16c + m (row permutation of the node kernels)
 __device__ __forceinline__ int rho(int c, int m) { return 32 * (c >> 1) + 8 * (m >> 2) + 4 * (c & 1) + (m & 3); }
 // first of the 4 features lane group g holds from tile c
@@ -1279,7 +1272,7 @@ int dmph_nn_gemm_head(const uint16_t *X, int ldx, const uint16_t *W, const float
 // features 8g .. 8g + 7 of row r, its MFMA fragment), h1 = relu(We1 x + be1),
 // h0 = We2 h1 + be2 (stored bf16 into H0, the left half of [h | S]) and
 // AB = Wab h0 (IEEE half), intermediates in registers (as k_node).
-template <bool FOLD>   // as k_node
+template <bool FOLD>   // as k_node (weights staged with rows in rho order)
 __global__ void __launch_bounds__(kNodeWaves * 64) k_enc(const float *__restrict__ vel_hist, const int *__restrict__ cls,
                                                          const float *__restrict__ pos, NnFeat ft,
                                                          const bf16_t *__restrict__ We1, const float *__restrict__ be1,
@@ -1292,18 +1285,16 @@ __global__ void __launch_bounds__(kNodeWaves * 64) k_enc(const float *__restrict
     const int lr = lane & 15, g = lane >> 4, lk = 8 * g;
     const int ntiles = (M + kNodeWaves * 16 - 1) / (kNodeWaves * 16);
     typedef uint4 frag_t;
-    for (int t = threadIdx.x; t < 128 * 32 / 8; t += blockDim.x) {   // We1 as is
+    for (int t = threadIdx.x; t < 128 * 32 / 8; t += blockDim.x) {   // We1, rows in rho order
         const int r = t / 4, c = (t % 4) * 8;
-        *reinterpret_cast<uint4 *>(we1 + wl_at(r, c, 32)) = *reinterpret_cast<const uint4 *>(We1 + (size_t)r * 32 + c);
+        *reinterpret_cast<uint4 *>(we1 + wl_at(r, c, 32)) =
+            *reinterpret_cast<const uint4 *>(We1 + (size_t)rho(r >> 4, r & 15) * 32 + c);
     }
-    for (int t = threadIdx.x; t < 384 * 128 / 8; t += blockDim.x) {   // We2 and Wab, columns in pk_col order
+    for (int t = threadIdx.x; t < 384 * 128 / 8; t += blockDim.x) {   // We2 and Wab, rows in rho order
         const int r = t / 16, c = (t % 16) * 8;
-        const bf16_t *src = r < 128 ? We2 + (size_t)r * 128 : Wab + (size_t)(r - 128) * 128;
-        bf16_t *dst = r < 128 ? we2 : wab;
-        const int rr = r < 128 ? r : r - 128;
-        const uint2 lo = *reinterpret_cast<const uint2 *>(src + pk_col(c));
-        const uint2 hi = *reinterpret_cast<const uint2 *>(src + pk_col(c + 4));
-        *reinterpret_cast<uint4 *>(dst + wl_at(rr, c, 128)) = make_uint4(lo.x, lo.y, hi.x, hi.y);
+        const int rr = r < 128 ? r : r - 128, q = rr & 127;
+        const bf16_t *src = (r < 128 ? We2 : Wab) + (size_t)((rr & ~127) + rho(q >> 4, q & 15)) * 128;
+        *reinterpret_cast<uint4 *>((r < 128 ? we2 : wab) + wl_at(rr, c, 128)) = *reinterpret_cast<const uint4 *>(src + c);
     }
     __syncthreads();
     for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
@@ -1343,7 +1334,7 @@ __global__ void __launch_bounds__(kNodeWaves * 64) k_enc(const float *__restrict
         uint32_t hf[8][2];
 #pragma unroll
         for (int c = 0; c < 8; c++) {   // h1 = relu(We1 x + be1)
-            const float4 b = *reinterpret_cast<const float4 *>(be1 + c * 16 + 4 * g);
+            const float4 b = *reinterpret_cast<const float4 *>(be1 + rho_base(c, g));
             hf[c][0] = pk_bf16_int(fmaxf(acc[c][0] + b.x, 0.f), fmaxf(acc[c][1] + b.y, 0.f));
             hf[c][1] = pk_bf16_int(fmaxf(acc[c][2] + b.z, 0.f), fmaxf(acc[c][3] + b.w, 0.f));
         }
@@ -1361,12 +1352,15 @@ __global__ void __launch_bounds__(kNodeWaves * 64) k_enc(const float *__restrict
         }
 #pragma unroll
         for (int c = 0; c < 8; c++) {
-            const float4 b = *reinterpret_cast<const float4 *>(be2 + c * 16 + 4 * g);
+            const float4 b = *reinterpret_cast<const float4 *>(be2 + rho_base(c, g));
             hf[c][0] = pk_bf16(acc[c][0] + b.x, acc[c][1] + b.y);
             hf[c][1] = pk_bf16(acc[c][2] + b.z, acc[c][3] + b.w);
-            if (r < M) *reinterpret_cast<uint2 *>(H0 + (size_t)r * ldh0 + c * 16 + 4 * g) = make_uint2(hf[c][0], hf[c][1]);
+            if ((c & 1) && r < M)
+                *reinterpret_cast<uint4 *>(H0 + (size_t)r * ldh0 + rho_base(c - 1, g)) =
+                    make_uint4(hf[c - 1][0], hf[c - 1][1], hf[c][0], hf[c][1]);
         }
         float px = 0.f, py = 0.f, pz = 0.f;
+        uint2 ab2[8];
 #pragma unroll
         for (int half = 0; half < 2; half++) {   // AB = Wab h0
             if (FOLD && half == 1 && r < M) {
@@ -1391,7 +1385,7 @@ __global__ void __launch_bounds__(kNodeWaves * 64) k_enc(const float *__restrict
 #pragma unroll
                 for (int c = 0; c < 8; c++) {
                     if (FOLD && half == 1) {   // + Wc3 P of the row
-                        const int col = c * 16 + 4 * g + (lb - lr);
+                        const int col = rho_base(c, g) + (lb - lr);
                         const float4 wx = *reinterpret_cast<const float4 *>(fold.Wt + col);
                         const float4 wy = *reinterpret_cast<const float4 *>(fold.Wt + 128 + col);
                         const float4 wz = *reinterpret_cast<const float4 *>(fold.Wt + 256 + col);
@@ -1400,11 +1394,11 @@ __global__ void __launch_bounds__(kNodeWaves * 64) k_enc(const float *__restrict
                         acc[c][2] += wx.z * px + wy.z * py + wz.z * pz;
                         acc[c][3] += wx.w * px + wy.w * py + wz.w * pz;
                     }
-                    uint32_t o[4];
-#pragma unroll
-                    for (int e = 0; e < 4; e++) o[e] = (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)acc[c][e]);
-                    *reinterpret_cast<uint2 *>(ABo + (size_t)r * 256 + half * 128 + c * 16 + 4 * g) =
-                        make_uint2(o[0] | o[1] << 16, o[2] | o[3] << 16);
+                    ab2[c] = make_uint2(__builtin_bit_cast(uint32_t, h2_t{(_Float16)acc[c][0], (_Float16)acc[c][1]}),
+                                        __builtin_bit_cast(uint32_t, h2_t{(_Float16)acc[c][2], (_Float16)acc[c][3]}));
+                    if (c & 1)
+                        *reinterpret_cast<uint4 *>(ABo + (size_t)r * 256 + half * 128 + rho_base(c - 1, g)) =
+                            make_uint4(ab2[c - 1].x, ab2[c - 1].y, ab2[c].x, ab2[c].y);
                 }
         }
     }
