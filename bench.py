@@ -353,7 +353,7 @@ def main(argv=None) -> int:
     ap.add_argument("--no-surrogate", action="store_true", help="skip the learned-surrogate measurement")
     ap.add_argument("--surrogate-chunk", type=int, default=96,
                     help="scenes per surrogate batch (96: ~7.8M nodes, within the 4 GB row offsets of the edge gathers)")
-    ap.add_argument("--surrogate-eval-scenes", type=int, default=16)
+    ap.add_argument("--surrogate-eval-scenes", type=int, default=64)
     ap.add_argument("--no-channel", action="store_true", help="skip the large-channel (C5) measurement")
     ap.add_argument("--channel-steps", type=int, default=100)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
