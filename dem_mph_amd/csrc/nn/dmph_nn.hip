@@ -19,6 +19,7 @@
 // All entry points are extern "C" with plain device pointers and a stream.
 #include <hip/hip_runtime.h>
 #include <algorithm>
+#include <mutex>
 #include <type_traits>
 #include <stdint.h>
 
@@ -1200,6 +1201,33 @@ using namespace nn;
 
 static int launch_ok() { return hipGetLastError() == hipSuccess ? 0 : -5; }
 
+// Loss partials of dmph_nn_mse_static: one float per 256-node block of k_mse,
+// in a file-scope device array instead of caller or library allocations.
+// Each device that runs the library holds its own copy until the process
+// ends; every call on one device shares it, so calls that overlap on one
+// device race.  k_mse writes every partial it launches before the reduction
+// reads it, so nothing needs clearing between calls.
+static __device__ __align__(256) float scratch[DMPH_NN_MSE_STATIC_MAX_N / 256];
+
+// Device address of `scratch` per device ordinal, looked up once per device
+// with that device current.  In host code `scratch` names a host-side stand-in,
+// never the device memory: kernels and copies get these addresses.
+static std::mutex g_scratch_mu;
+static float *g_scratch_ptr[256];
+
+static int static_scratch(float **out) {
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 256) return -5;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return -5;
+        g_scratch_ptr[dev] = (float *)p;
+    }
+    *out = g_scratch_ptr[dev];
+    return 0;
+}
+
 extern "C" {
 
 int dmph_nn_gemm(const uint16_t *X, int ldx, const uint16_t *W, const float *bias, const uint16_t *R, int ldr,
@@ -1681,6 +1709,19 @@ int dmph_nn_mse(const float *Y, const float *T, const int *mask, int N, int coun
     const int g = (N + 255) / 256;
     hipLaunchKernelGGL(k_mse, dim3(g), dim3(256), 0, s, Y, T, mask, N, 1.f / (float)count, clip, dY, scratch);
     reduce_partials(scratch, g, 1, 1, loss, 0, s);
+    return launch_ok();
+}
+
+int dmph_nn_mse_static(const float *Y, const float *T, const int *mask, int N, int count, float clip, float *dY,
+                       float *loss, void *stream) {
+    if (N <= 0 || count <= 0 || !(clip > 0.f)) return -1;
+    if (N > DMPH_NN_MSE_STATIC_MAX_N) return -4;   // the partials would not fit the static array
+    float *part = nullptr;
+    if (static_scratch(&part) != 0) return -5;
+    hipStream_t s = (hipStream_t)stream;
+    const int g = (N + 255) / 256;
+    hipLaunchKernelGGL(k_mse, dim3(g), dim3(256), 0, s, Y, T, mask, N, 1.f / (float)count, clip, dY, part);
+    reduce_partials(part, g, 1, 1, loss, 0, s);
     return launch_ok();
 }
 

@@ -6,7 +6,9 @@
  * plain solver; dem_mph_amd/surrogate.py drives them.  All pointers are device
  * pointers; `stream` is a hipStream_t.  bf16 values are raw 16-bit storage.
  * Weights of a layer with K inputs and N outputs are [N][K].  Calls return 0,
- * -1 for unsupported shapes, -2 when a capacity is too small, -5 on a HIP error.
+ * -1 for unsupported shapes, -2 when a capacity is too small, -3 when a graph
+ * would overflow int32 row offsets, -4 when a size exceeds a fixed array of the
+ * library, -5 on a HIP error.
  */
 #ifndef DEM_MPH_NN_H
 #define DEM_MPH_NN_H
@@ -174,6 +176,19 @@ int dmph_nn_head_bwd(const uint16_t *H, const float *Wd, const float *dY, uint16
 size_t dmph_nn_mse_scratch(int N);
 int dmph_nn_mse(const float *Y, const float *T, const int *mask, int N, int count, float clip, float *dY, float *loss,
                 float *scratch, void *stream);
+
+/* dmph_nn_mse with no scratch argument and no allocation of any kind: the
+ * per-block loss partials live in a fixed device array inside the library
+ * (256 KB per device, held until the process ends), so N is at most
+ * DMPH_NN_MSE_STATIC_MAX_N; larger N returns -4 and launches nothing.  No
+ * counterpart in the reference: this entry is for callers that cannot
+ * allocate device memory (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc).  Every call on one device shares the one
+ * array, so calls that overlap on a device (two streams, two threads) race;
+ * serialise them.  Same results as dmph_nn_mse, bit for bit. */
+#define DMPH_NN_MSE_STATIC_MAX_N (65536 * 256)
+int dmph_nn_mse_static(const float *Y, const float *T, const int *mask, int N, int count, float clip, float *dY,
+                       float *loss, void *stream);
 
 int dmph_nn_relu_bwd(const uint16_t *Y, int ldy, const uint16_t *dY, int lddy, uint16_t *dX, int lddx, int M, int n,
                      void *stream);
