@@ -99,6 +99,7 @@ struct Dev {
     int *b_moff, *b_mslot;         // per body: offsets into b_mslot [nb + 1]; member slots [members], by rank
     double *blk_max;               // per-block maxima of the squared displacement
     float4 *p4, *v4;               // FP32 sweeps: packed (position, type) and (velocity, PressureP)
+    const double *x64;             // FP32, first binning only: the input positions [n][3] in original order
     Scalars *sc;
 };
 

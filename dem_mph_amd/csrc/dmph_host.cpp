@@ -348,10 +348,23 @@ class Engine final : public dmph_handle {
         for (int w = 0; w < 2; w++)
             for (int k = 0; k < 3; k++) sc.wall_center[w][k] = p.wall_center[w + 4][k];
         HCHK(hipMemcpyAsync(a.sc, &sc, sizeof sc, hipMemcpyHostToDevice, stream));
+        // FP32: the first rows are decided on the input positions (as the FP64
+        // path decides them), not on their FP32 offsets; the copy is freed after
+        double *x64 = nullptr;
+        if (!std::is_same<R, double>::value) {
+            HCHK(hipMalloc(&x64, (size_t)n * 3 * sizeof(double)));
+            const hipError_t e = hipMemcpyAsync(x64, gr->pos, (size_t)n * 3 * sizeof(double), hipMemcpyHostToDevice,
+                                                stream);
+            if (e != hipSuccess) { (void)hipFree(x64); HCHK(e); }
+            a.x64 = x64;
+        }
         launch_rebin<R>(g, a, stream);
         launch_init_density<R>(g, a, stream);
-        HCHK(hipGetLastError());
-        HCHK(hipStreamSynchronize(stream));
+        a.x64 = nullptr;
+        const hipError_t e1 = hipGetLastError(), e2 = hipStreamSynchronize(stream);
+        if (x64) (void)hipFree(x64);
+        HCHK(e1);
+        HCHK(e2);
         return DMPH_OK;
     }
 

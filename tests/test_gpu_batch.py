@@ -12,25 +12,29 @@ pytestmark = pytest.mark.gpu
 
 def _batch(n, seed, rigid):
     scenes = S.ensemble_c3(n, seed=seed, d=0.02, interior=(0.6, 0.3, 0.4), fluid_range=(300.0, 3000.0))
-    if not rigid:   # loose grains: the FP64 rigid-body sums use atomics, whose order is not fixed
+    if not rigid:   # loose DEM grains, as the reference moves its solid particles
         for sc in scenes:
             sc.body = None
             sc.density = None
     return scenes
 
 
-def test_batch_fp64_equals_each_scene_alone_bit_for_bit():
+@pytest.mark.parametrize("rigid", [False, True])
+def test_batch_fp64_equals_each_scene_alone_bit_for_bit(rigid):
     """FP64, canonical order, rebinning every step: every field of every scene
-    of the batch equals the scene's own handle bit for bit."""
-    scenes = _batch(3, 2, rigid=False)
+    of the batch equals the scene's own handle bit for bit, and with rigid
+    debris so does every body's state (one block per body sums its members in
+    a fixed order, so the sums do not depend on the batch around them)."""
+    scenes = _batch(3, 2, rigid=rigid)
     assert len({sc.n for sc in scenes}) > 1
     p = D.Params.from_dict(S.params_c1())
-    kw = dict(precision=D.PREC_FP64, sort_mode=D.SORT_CANONICAL, rebuild=D.REBUILD_EVERY_STEP)
+    kw = dict(precision=D.PREC_FP64, sort_mode=D.SORT_CANONICAL, rebuild=D.REBUILD_EVERY_STEP, rigid=rigid)
     batch = D.Solver(p, scenes, **kw)
     alone = [D.Solver(p, sc, **kw) for sc in scenes]
     info = batch.info()
     assert info.nscenes == 3 and info.n == sum(sc.n for sc in scenes)
     assert info.cells == alone[0].info().cells
+    assert info.nbodies == sum(s.info().nbodies for s in alone) and (info.nbodies > 0) == rigid
     batch.step(6)
     for s in alone:
         s.step(6)
@@ -43,7 +47,60 @@ def test_batch_fp64_equals_each_scene_alone_bit_for_bit():
             if f == "id":
                 ref = ref + off[k]
             np.testing.assert_array_equal(fb[off[k]:off[k + 1]], ref, err_msg="%s scene %d" % (f, k))
+    if rigid:
+        bb = batch.bodies()
+        b0 = 0
+        for k, s in enumerate(alone):
+            ba = s.bodies()
+            nb = len(ba["mass"])
+            for f in ("x", "v", "q", "l", "mass"):
+                np.testing.assert_array_equal(bb[f][b0:b0 + nb], ba[f], err_msg="body %s scene %d" % (f, k))
+            b0 += nb
     assert batch.time() == alone[0].time()
+
+
+def test_ensemble_c3_full_size():
+    """The 512-scene C3 ensemble (39.9M particles, 8,315 rigid bodies) in one
+    handle, FP32 pairs with FP64 rigid bodies, rebinning every step, 5 steps.
+    Size-independent properties: state finite, the block counts and every body
+    mass unchanged, the clock exactly 5 dt, every scene's particles still in
+    its own id range; and sampled scenes (first, last and two inside) equal,
+    bit for bit, the same scene advanced in a handle of its own (a scene's rows,
+    sums and body updates do not depend on the batch around it)."""
+    scenes = S.ensemble_c3(512, seed=1)
+    params = S.params_c3()
+    kw = dict(precision=D.PREC_FP32, rigid=True, rebuild=D.REBUILD_EVERY_STEP)
+    print("C3: %d scenes, %d particles" % (len(scenes), sum(sc.n for sc in scenes)), flush=True)
+    batch = D.Solver(D.Params.from_dict(params), scenes, **kw)
+    info = batch.info()
+    assert info.nscenes == 512 and info.n == sum(sc.n for sc in scenes)
+    m0 = batch.bodies()["mass"].copy()
+    assert len(m0) == info.nbodies == sum(int(sc.body.max()) + 1 for sc in scenes if (sc.body >= 0).any())
+    batch.step(5)
+    batch.sync()
+    pos, vel = batch.field("pos"), batch.field("vel")
+    assert np.isfinite(pos).all() and np.isfinite(vel).all()
+    t = 0.0
+    for _ in range(5):   # the FP64 accumulator: 5 additions of dt from 0
+        t += params["Dt"]
+    assert batch.time() == t
+    typ = batch.field("type", D.ORDER_SLOT)
+    allt = np.concatenate([sc.type for sc in scenes])
+    np.testing.assert_array_equal(np.bincount(typ, minlength=6), np.bincount(allt, minlength=6))
+    np.testing.assert_array_equal(batch.bodies()["mass"], m0)
+    off = batch.scene_offsets
+    ids = batch.field("id", D.ORDER_SLOT)
+    key = batch.field("key", D.ORDER_SLOT)
+    scene_of_slot = (key % (512 * info.cells)) // info.cells
+    np.testing.assert_array_equal(np.searchsorted(off, ids, side="right") - 1, scene_of_slot)
+    for k in (0, 137, 300, 511):
+        s = D.Solver(D.Params.from_dict(params), scenes[k], **kw)
+        s.step(5)
+        s.sync()
+        np.testing.assert_array_equal(pos[off[k]:off[k + 1]], s.field("pos"), err_msg="pos scene %d" % k)
+        np.testing.assert_array_equal(vel[off[k]:off[k + 1]], s.field("vel"), err_msg="vel scene %d" % k)
+        s.close()
+    batch.close()
 
 
 def test_batch_fp32_rigid_within_stated_tolerance_of_oracle():
