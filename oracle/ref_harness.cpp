@@ -91,8 +91,9 @@ static void load(char* data, char* grid, const char* log) {
     g_dem_substeps = Dt / DEMDt;   // source/DEM-MPH.cpp:434 (truncating conversion)
 }
 
-// One pass of the main loop body, source/DEM-MPH.cpp:450-579, without file output.
-static void run_step() {
+// One pass of the main loop body, source/DEM-MPH.cpp:450-579, without file
+// output and without the time advance.
+static void run_step_body() {
     calculateConvection();
     calculateWall();
     calculatePeriodicBoundary();
@@ -125,7 +126,43 @@ static void run_step() {
         calculateTangentialDirectionForce();
         calculateDEMconvection();
     }
+}
+
+static void run_step() {
+    run_step_body();
     Time += Dt;
+}
+
+// The reference's main loop with its file output, source/DEM-MPH.cpp:419-580:
+// the initial output.vtk, then per step the .prof written before the step and
+// the virial pass + .vtk after it, each when its next output time is reached,
+// file names from sprintf(format, iStep) with iStep = (int)(Time / Dt) at the
+// start, and the "@ ... Output Time" log lines.  Everything it calls is the
+// reference's own code; only acc_set_device_num and the clock() buckets are
+// left out.
+static void run_files(char* proffmt, char* vtkfmt) {
+    writeVtkFile((char*)"output.vtk");
+    int iStep = (int)(Time / Dt);
+    while (Time < EndTime + 1.0e-5 * Dt) {
+        if (Time + 1.0e-5 * Dt >= OutputNext) {
+            char filename[256];
+            sprintf(filename, proffmt, iStep);
+            writeProfFile(filename);
+            log_printf("@ Prof Output Time : %e\n", Time);
+            OutputNext += OutputInterval;
+        }
+        run_step_body();
+        if (Time + 1.0e-5 * Dt >= VtkOutputNext) {
+            calculateVirialStressAtParticle();
+            char filename[256];
+            sprintf(filename, vtkfmt, iStep);
+            writeVtkFile(filename);
+            log_printf("@ Vtk Output Time : %e\n", Time);
+            VtkOutputNext += VtkOutputInterval;
+        }
+        Time += Dt;
+        iStep++;
+    }
 }
 
 static void dump(const char* path, int step, bool virial) {
@@ -193,8 +230,16 @@ static void dump(const char* path, int step, bool virial) {
 //   writes <out_prefix>_<step>.bin after initialisation (step 0), every
 //   dump_every steps and after the last step.  With --time it instead times the
 //   step loop (wall clock) and prints one line "steps=<k> seconds=<s>".
+// usage: refNd --files <data> <grid> <prof_format> <vtk_format> <log>
+//   the reference's program as it runs, file output included (run_files).
 int main(int argc, char** argv) {
     using namespace dmph_ref_harness;
+    if (argc == 7 && !strcmp(argv[1], "--files")) {
+        load(argv[2], argv[3], argv[6]);
+        run_files(argv[4], argv[5]);
+        log_close();
+        return 0;
+    }
     if (argc < 5) {
         fprintf(stderr, "usage: %s data grid out_prefix nsteps [dump_every] [--time]\n", argv[0]);
         return 2;
