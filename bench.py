@@ -31,16 +31,25 @@ import numpy as np  # noqa: E402
 HBM_PEAK_GBS = 8000.0
 METRIC = "particle-steps/s and sim-s/wall-s on 1 MI355X, plain vs mixed/batched/learned"
 
-# Algorithmic HBM bytes per particle per launch (FP32 state): each slot's own
-# inputs read once and outputs written once; neighbour gathers re-read arrays
-# already counted and are served from cache in the ideal.
-#   sweep1: pos 12 + vel 12 + type 4 + key 4 + rho 4 read; VolStrainP, DivergenceP,
-#           PressureA, mass, kappa, PressureP written (6 x 4)            = 36 + 24 = 60
-#   sweep2 (fluid + solid slots): pos 12 + vel 12 + type 4 + key 4 + PressureP 4
-#           + rho 4 + mass 4 read; force 12 written                      = 44 + 12 = 56
+# Algorithmic HBM bytes per unit and launch (FP32 state), the per-particle
+# contract figures of the step: each unit's own inputs read once and outputs
+# written once; neighbour gathers re-read arrays already counted and are
+# served from cache in the ideal.
+#   sweep1 (every particle): P-density, divergence, A-density, GC, alpha
+#           32 read + 28 written                                           = 60
+#   sweep2 (fluid + solid slots): all forces and the fused integration
+#           56 read + 28 written (force 12, velocity 12, PressureP copy 4) = 84
 #   dem (solid + wall slots): pos 12 + vel 12 + type 4 + key 4 + body 4 + force 12
 #           + four coefficients 16 read; force 12 + coefficients 16 written = 64 + 28 = 92
-ALGO_BYTES = {"sweep1": 60.0, "sweep2": 56.0, "dem": 92.0}
+# The whole step: 244 B per particle (binning 24, rank/scatter 8, reorder 72,
+# sweep 1 60, sweep 2 84 -- the last two counted for every particle).
+ALGO_BYTES = {"sweep1": 60.0, "sweep2": 84.0, "dem": 92.0}
+STEP_BYTES = 244.0
+# The CPU baseline is this package's FP64 restatement (oracle/); the reference
+# solver built from its own sources ran slower on the same input files in the
+# build container (8 threads, oracle/calibrate_cpu.py): C2 8.48 s/step against
+# 4.39 s/step, C1 171 ms/step against 97 ms/step.
+PORT_OVER_REFERENCE = {"dam_break_c2": 1.93, "dam_break_c1": 1.77}
 
 
 def build_workload(seed: int = 1):
@@ -75,7 +84,10 @@ def cpu_baseline(sc, params, budget_s: float = 20.0):
             break
     rec = {"value": sc.n * steps / el, "unit": "particle-steps/s", "cores": threads, "kind": "port",
            "sample": "FP64 oracle, rigid debris, same %d-particle scene, %d steps after set-up (%.1f s)"
-                     % (sc.n, steps, el)}
+                     % (sc.n, steps, el),
+           "port_over_reference_speed": PORT_OVER_REFERENCE["dam_break_c2"],
+           "calibration": "the reference built from its own sources ran this scene (loose grains) at 8.48 s/step, "
+                          "the port at 4.39 s/step, 8 threads each, same files (oracle/calibrate_cpu.py)"}
     return rec, o, steps
 
 
@@ -90,10 +102,18 @@ def balance(mass, vel, typ):
     return float(mass[mv].sum()), (mass[mv, None] * vel[mv]).sum(axis=0)
 
 
-def errors_vs_oracle(D, sc, params, prec, compensated, o, steps):
+# the solver variants timed on C2: options of dem_mph_amd.Solver
+VARIANTS = {
+    "mixed": dict(precision=1, compensated=True, rigid_precision=0),    # FP32 pairs, FP64 sums, FP64 bodies
+    "plain": dict(precision=1, compensated=False, rigid_precision=1),   # FP32 everywhere but the clock
+    "fp64": dict(precision=0),                                           # FP64 everywhere (the parity mode)
+}
+
+
+def errors_vs_oracle(D, sc, params, variant, o, steps):
     """Per-field errors of a fresh GPU run of `steps` steps against the FP64
     oracle at the same step, plus mass and momentum balance of both."""
-    s = D.Solver(D.Params.from_dict(params), sc, precision=prec, rigid=True, compensated=compensated)
+    s = D.Solver(D.Params.from_dict(params), sc, rigid=True, **VARIANTS[variant])
     s.step(steps)
     s.sync()
     typ = o.field("type")
@@ -149,16 +169,16 @@ def kernel_traffic(kernel_name: str):
     return sum(hits) if hits else None
 
 
-def time_variant(D, sc, params, prec, compensated, steps, warmup):
+def time_variant(D, sc, params, variant, steps, warmup):
     """Timed region: `steps` replays of the captured step (hipGraph), bracketed
     by device synchronisation.  Then the same number of steps launched one by
     one with HIP events around each kernel group, on the library's stream, for
     the per-kernel durations."""
-    s = D.Solver(D.Params.from_dict(params), sc, precision=prec, rigid=True, compensated=compensated, graph=True)
+    s = D.Solver(D.Params.from_dict(params), sc, rigid=True, graph=True, **VARIANTS[variant])
     info = s.info()
     s.step(warmup)
     s.sync()
-    lanes = s.autotune(5)   # lanes per particle of the two sweeps, timed on this scene
+    lanes = s.autotune(5)   # lanes per particle and workgroup size of the two sweeps, timed on this scene
     s.step(warmup)
     s.sync()
     reb0 = s.info().rebuilds
@@ -174,9 +194,31 @@ def time_variant(D, sc, params, prec, compensated, steps, warmup):
     el_ev = time.perf_counter() - t1
     tm = s.timing()
     s.set_timing(False)
+    blocks = list(s.info().block)
     s.close()
     return {"seconds": el, "timing": tm, "info": info, "rebuilds": int(reb - reb0), "seconds_timed_launches": el_ev,
-            "lanes": list(lanes)}
+            "lanes": list(lanes), "blocks": blocks}
+
+
+def full_config(D, sc, params, variant, steps):
+    """The whole C2 configuration as stated: `steps` steps from t = 0 (every
+    rebuild of the run included), after autotuning on the initial state."""
+    s = D.Solver(D.Params.from_dict(params), sc, rigid=True, graph=True, **VARIANTS[variant])
+    lanes = s.autotune(3)
+    s.step(1)   # the graph capture, outside the clock; the run below starts at step 1
+    s.sync()
+    reb0 = s.info().rebuilds
+    t0 = time.perf_counter()
+    s.step(steps - 1)
+    s.sync()
+    el = time.perf_counter() - t0
+    reb = s.info().rebuilds - reb0
+    t_end = s.time()
+    s.close()
+    dt = params["Dt"]
+    return {"steps": steps, "timed_steps": steps - 1, "ms_per_step": el / (steps - 1) * 1e3,
+            "particle_steps_per_s": sc.n * (steps - 1) / el, "sim_s_per_wall_s": dt * (steps - 1) / el,
+            "rebuilds": int(reb), "simulated_s": t_end, "autotuned_lanes": list(lanes)}
 
 
 def ensemble_bench(D, nscenes: int, steps: int, warmup: int, seed: int, sample: int):
@@ -194,9 +236,9 @@ def ensemble_bench(D, nscenes: int, steps: int, warmup: int, seed: int, sample: 
                        "1-32 rigid cuboids of side 3-8, dt = 5e-5 s" % nscenes,
            "scenes": nscenes, "particles": n, "steps": steps, "generation_s": t_gen}
     res = {}
-    for name, comp in (("batched_mixed", True), ("batched_plain", False)):
+    for name, var in (("batched_mixed", "mixed"), ("batched_plain", "plain")):
         t0 = time.perf_counter()
-        s = D.Solver(D.Params.from_dict(params), scenes, precision=D.PREC_FP32, rigid=True, compensated=comp)
+        s = D.Solver(D.Params.from_dict(params), scenes, rigid=True, **VARIANTS[var])
         t_create = time.perf_counter() - t0
         s.step(warmup)
         s.sync()
@@ -211,13 +253,16 @@ def ensemble_bench(D, nscenes: int, steps: int, warmup: int, seed: int, sample: 
         res[name] = {"particle_steps_per_s": n * steps / el, "ms_per_step": el / steps * 1e3,
                      "sim_s_per_wall_s": dt * steps / el, "create_s": t_create, "autotuned_lanes": list(lanes)}
         print("bench: ensemble %s %.2f ms/step" % (name, el / steps * 1e3), file=sys.stderr, flush=True)
-    # plain FP32 solver, one scene per handle, scenes one after another
+    # plain FP32 solver, one scene per handle, scenes one after another (each autotuned)
     idx = np.linspace(0, nscenes - 1, min(sample, nscenes)).round().astype(int)
     el = 0.0
     n_s = 0
     for k in idx:
-        s = D.Solver(D.Params.from_dict(params), scenes[k], precision=D.PREC_FP32, rigid=True, compensated=False)
+        s = D.Solver(D.Params.from_dict(params), scenes[k], rigid=True, **VARIANTS["plain"])
         s.step(warmup)
+        s.sync()
+        s.autotune(2)
+        s.step(2)
         s.sync()
         t0 = time.perf_counter()
         s.step(steps)
@@ -229,10 +274,13 @@ def ensemble_bench(D, nscenes: int, steps: int, warmup: int, seed: int, sample: 
     el_all = el * scale
     res["sequential_plain"] = {"particle_steps_per_s": n * steps / el_all, "ms_per_step": el_all / steps * 1e3,
                                "sim_s_per_wall_s": dt * steps / el_all,
-                               "sample": "%d of %d scenes (%d particles), time scaled by particle count"
-                                         % (len(idx), nscenes, n_s)}
+                               "sample": "%d of %d scenes (%d particles), each autotuned, time scaled by particle "
+                                         "count" % (len(idx), nscenes, n_s)}
     out.update(res)
-    out["batched_over_sequential"] = res["batched_mixed"]["sim_s_per_wall_s"] / res["sequential_plain"]["sim_s_per_wall_s"]
+    out["batched_mixed_over_sequential"] = (res["batched_mixed"]["sim_s_per_wall_s"] /
+                                            res["sequential_plain"]["sim_s_per_wall_s"])
+    out["batched_mixed_over_batched_plain"] = (res["batched_mixed"]["sim_s_per_wall_s"] /
+                                               res["batched_plain"]["sim_s_per_wall_s"])
     return out, scenes
 
 
@@ -343,7 +391,11 @@ def main(argv=None) -> int:
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--precision", choices=("fp32", "fp64"), default="fp32")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-plain", action="store_true", help="skip the plain variant (FP32 sums)")
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain variant (FP32 throughout)")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 variant")
+    ap.add_argument("--fp64-steps", type=int, default=100)
+    ap.add_argument("--config-steps", type=int, default=2000,
+                    help="steps of the whole-configuration run from t = 0 (0: skip)")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble measurement")
@@ -369,15 +421,18 @@ def main(argv=None) -> int:
         copy_probe(a.copy_probe)
     sc, params = build_workload(a.seed)
     prec = D.PREC_FP32 if a.precision == "fp32" else D.PREC_FP64
+    head = "mixed" if prec == D.PREC_FP32 else "fp64"
     n = sc.n
     dt = params["Dt"]
-    print("bench: %d particles, %s, mixed precision (FP64 sums, FP64 rigid bodies)" % (n, a.precision),
-          file=sys.stderr, flush=True)
-    mixed = time_variant(D, sc, params, prec, True, a.steps, a.warmup)
-    variants = {"mixed": mixed}
+    print("bench: %d particles, headline variant %s" % (n, head), file=sys.stderr, flush=True)
+    mixed = time_variant(D, sc, params, head, a.steps, a.warmup)
+    variants = {head: mixed}
     if not a.no_plain and prec == D.PREC_FP32:
-        print("bench: plain FP32 sums", file=sys.stderr, flush=True)
-        variants["plain"] = time_variant(D, sc, params, prec, False, a.steps, a.warmup)
+        print("bench: plain FP32 (sums and rigid bodies in FP32)", file=sys.stderr, flush=True)
+        variants["plain"] = time_variant(D, sc, params, "plain", a.steps, a.warmup)
+    if not a.no_fp64 and prec == D.PREC_FP32:
+        print("bench: FP64 (the parity mode)", file=sys.stderr, flush=True)
+        variants["fp64"] = time_variant(D, sc, params, "fp64", a.fp64_steps, a.warmup)
     info = mixed["info"]
     el = mixed["seconds"]
     pss = n * a.steps / el
@@ -391,6 +446,7 @@ def main(argv=None) -> int:
     # the kernels the autotuner picked (the profile holds every candidate's launches)
     lanes = mixed["lanes"]
     kname = {"sweep1": "k_sweep1f<%d, " % lanes[0], "sweep2": "k_sweep2f<%d, " % lanes[1], "dem": "k_dem<float"}[dom]
+    step_achieved = pss * STEP_BYTES / 1e9
     out = {
         "metric": METRIC,
         "value": pss,
@@ -413,21 +469,29 @@ def main(argv=None) -> int:
         "rebuilds_in_timed_region": mixed["rebuilds"],
         "kernel_ms_per_step": {k: v[0] / a.steps for k, v in groups.items()},
         "ms_per_step_timed_launches": mixed["seconds_timed_launches"] / a.steps * 1e3,
-        "variants": {k: {"particle_steps_per_s": n * a.steps / v["seconds"], "ms_per_step": v["seconds"] / a.steps * 1e3,
-                         "sim_s_per_wall_s": dt * a.steps / v["seconds"], "autotuned_lanes": v["lanes"]}
+        "variants": {k: {"particle_steps_per_s": n * (a.fp64_steps if k == "fp64" else a.steps) / v["seconds"],
+                         "ms_per_step": v["seconds"] / (a.fp64_steps if k == "fp64" else a.steps) * 1e3,
+                         "sim_s_per_wall_s": dt * (a.fp64_steps if k == "fp64" else a.steps) / v["seconds"],
+                         "steps": a.fp64_steps if k == "fp64" else a.steps,
+                         "options": {kk: vv for kk, vv in VARIANTS[k].items()},
+                         "autotuned_lanes": v["lanes"], "autotuned_blocks": v["blocks"]}
                      for k, v in variants.items()},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": kernel_traffic(kname),
                      "algorithmic_bytes_per_unit": ALGO_BYTES[dom], "units_per_launch": units,
-                     "avg_launch_ms": avg_s * 1e3},
+                     "avg_launch_ms": avg_s * 1e3,
+                     "whole_step": {"algorithmic_bytes_per_particle": STEP_BYTES, "achieved": step_achieved,
+                                    "frac": step_achieved / HBM_PEAK_GBS}},
     }
+    if a.config_steps > 0 and prec == D.PREC_FP32:
+        print("bench: the whole C2 configuration, %d steps from t = 0" % a.config_steps, file=sys.stderr, flush=True)
+        out["full_config"] = {v: full_config(D, sc, params, v, a.config_steps) for v in ("mixed", "plain")}
     if not a.no_cpu_baseline:
         print("bench: timing the CPU baseline (bounded sample)", file=sys.stderr, flush=True)
         rec, o, k = cpu_baseline(sc, params, a.cpu_budget)
         out["cpu_baseline"] = rec
         print("bench: errors against the FP64 oracle after %d steps" % k, file=sys.stderr, flush=True)
-        out["errors_vs_fp64_oracle"] = {
-            v: errors_vs_oracle(D, sc, params, prec, v == "mixed", o, k) for v in variants}
+        out["errors_vs_fp64_oracle"] = {v: errors_vs_oracle(D, sc, params, v, o, k) for v in variants}
         o.close()
     else:
         out["cpu_baseline"] = None
@@ -438,7 +502,7 @@ def main(argv=None) -> int:
         if not a.no_surrogate:
             print("bench: learned surrogate", file=sys.stderr, flush=True)
             ens["surrogate_plus_fallback"] = surrogate_bench(
-                D, scenes, ens["sequential_plain"]["ms_per_step"], ens["batched_mixed"]["ms_per_step"],
+                D, scenes, ens["sequential_plain"]["ms_per_step"], ens["batched_plain"]["ms_per_step"],
                 a.surrogate_chunk, 3, a.surrogate_eval_scenes, 100, a.seed)
         del scenes
     if not a.no_channel:

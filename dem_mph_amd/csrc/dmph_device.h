@@ -41,9 +41,11 @@ struct Geo {
     int kcap;           // neighbour-list capacity per particle
     int knear;          // FP32 rows: near segment capacity (far entries start at this position), a multiple of 16
     int lanes[2];       // FP32 sweeps 1 and 2: lanes per particle (2, 4, 8 or 16)
+    int block[2];       // FP32 sweeps 1 and 2: threads per workgroup (64, 128 or 256)
     int fuse_accel;     // FP32: the fluid sweep 2 also applies calculateAcceleration (0 while re-running sweeps)
     int nloose;         // solid particles outside rigid bodies (moved by k_move_solids)
     int ccap;           // DEM contact-list capacity per particle
+    int body_f32;       // FP32 only: rigid-body sums and integration in FP32 (the plain solver)
     double cw[3], dmin[3], dw[3];
     double d, vol, dt, dem_dt;
     double ra, rg, rp, rv, rmax, cut2, skin2;

@@ -155,7 +155,9 @@ static void walls(Geo &g, const dmph_params &p) {
 }
 
 // initializeDomain (source/DEM-MPH.cpp:1207-1262)
-static int domain(Geo &g, const dmph_grid &gr, std::string &err) {
+// initializeDomain (source/DEM-MPH.cpp:1207-1262); cell_scale > 1 (FP32 only)
+// widens the cells beyond MaxRadius + 0.1 d
+static int domain(Geo &g, const dmph_grid &gr, double cell_scale, std::string &err) {
     double m = 0.0;
     m = g.ra > m ? g.ra : m;
     m = g.rg > m ? g.rg : m;
@@ -168,9 +170,10 @@ static int domain(Geo &g, const dmph_grid &gr, std::string &err) {
         g.dmin[a] = gr.domain_min[a];
         g.dw[a] = gr.domain_max[a] - gr.domain_min[a];
     }
-    cnt[0] = floor((gr.domain_max[0] - gr.domain_min[0]) / (g.rmax + margin));
-    cnt[1] = floor((gr.domain_max[1] - gr.domain_min[1]) / (g.rmax + margin));
-    cnt[2] = g.dim == 2 ? 1 : floor((gr.domain_max[2] - gr.domain_min[2]) / (g.rmax + margin));
+    const double cmin = cell_scale > 1.0 ? cell_scale * (g.rmax + margin) : g.rmax + margin;
+    cnt[0] = floor((gr.domain_max[0] - gr.domain_min[0]) / cmin);
+    cnt[1] = floor((gr.domain_max[1] - gr.domain_min[1]) / cmin);
+    cnt[2] = g.dim == 2 ? 1 : floor((gr.domain_max[2] - gr.domain_min[2]) / cmin);
     for (int a = 0; a < 3; a++) g.cc[a] = (int)cnt[a];
     if (g.cc[0] < 3 || g.cc[1] < 3 || (g.dim == 3 && g.cc[2] < 3)) {
         char b[200];
@@ -217,6 +220,7 @@ class Engine final : public dmph_handle {
     std::vector<void *> allocs;
     int device = 0;
     double spacing = 0;
+    double cell_scale = 1.0;
     double dmax[3] = {0, 0, 0};
 
     ~Engine() override {
@@ -276,13 +280,21 @@ class Engine final : public dmph_handle {
         g.substeps = (int)(p.dt / p.dem_dt);   // source/DEM-MPH.cpp:434
         g.compensated = o.compensated;
         g.lanes[0] = g.lanes[1] = (o.lanes == 2 || o.lanes == 4 || o.lanes == 16) ? o.lanes : 8;
+        g.block[0] = g.block[1] = (o.block == 64 || o.block == 128) ? o.block : 256;
+        g.body_f32 = o.rigid_precision == DMPH_PREC_FP32;
         g.fuse_accel = 1;
         g.sort_reference = o.sort_mode == DMPH_SORT_REFERENCE;
         g.rebuild_every = o.rebuild == DMPH_REBUILD_EVERY_STEP;
         weights(g, p);
         tables(g, p);
         walls(g, p);
-        TRY(domain(g, *gr, err));
+        const bool fp32 = !std::is_same<R, double>::value;
+        cell_scale = o.cell_scale > 1.0 ? o.cell_scale : 1.0;
+        if (cell_scale > 1.0 && (!fp32 || cell_scale > 2.0)) {
+            err = "cell_scale above 1 needs FP32 and at most 2 (FP64 keeps the reference's cell grid)";
+            return DMPH_E_ARG;
+        }
+        TRY(domain(g, *gr, cell_scale, err));
         int pw = 0;
         while ((n >> pw) != 0) ++pw;
         g.p2_log = pw;
@@ -643,30 +655,31 @@ class Engine final : public dmph_handle {
         HCHK(hipMemcpyAsync(save + 2 * n, a.fz, n * sizeof(R), hipMemcpyDeviceToDevice, stream));
         hipEvent_t e0 = take_event(), e1 = take_event();
         if (!e0 || !e1) { err = "hipEventCreate failed"; return DMPH_E_DEVICE; }
-        const int cand[4] = {2, 4, 8, 16};
-        const int orig[2] = {g.lanes[0], g.lanes[1]};
-        int picks[2] = {orig[0], orig[1]};
+        const int cand[4] = {2, 4, 8, 16}, bcand[3] = {64, 128, 256};
+        const int orig[2] = {g.lanes[0], g.lanes[1]}, borig[2] = {g.block[0], g.block[1]};
+        int picks[2] = {orig[0], orig[1]}, bpicks[2] = {borig[0], borig[1]};
         for (int w = 0; w < 2; w++) {
             float best = 1e30f;
-            int pick = 8;
-            for (int c = 0; c < 4; c++) {
-                g.lanes[w] = cand[c];
-                auto launch = [&] {
-                    if (w == 0) launch_sweep1<R>(g, a, stream);
-                    else launch_sweep2<R>(g, a, stream);
-                };
-                launch();
-                HCHK(hipEventRecord(e0, stream));
-                for (int r = 0; r < reps; r++) launch();
-                HCHK(hipEventRecord(e1, stream));
-                HCHK(hipEventSynchronize(e1));
-                float ms = 0.f;
-                HCHK(hipEventElapsedTime(&ms, e0, e1));
-                tune_ms[w][c] = ms / reps;
-                if (ms < best) { best = ms; pick = cand[c]; }
-            }
-            picks[w] = pick;
+            for (int c = 0; c < 4; c++)
+                for (int bc = 0; bc < 3; bc++) {
+                    g.lanes[w] = cand[c];
+                    g.block[w] = bcand[bc];
+                    auto launch = [&] {
+                        if (w == 0) launch_sweep1<R>(g, a, stream);
+                        else launch_sweep2<R>(g, a, stream);
+                    };
+                    launch();
+                    HCHK(hipEventRecord(e0, stream));
+                    for (int r = 0; r < reps; r++) launch();
+                    HCHK(hipEventRecord(e1, stream));
+                    HCHK(hipEventSynchronize(e1));
+                    float ms = 0.f;
+                    HCHK(hipEventElapsedTime(&ms, e0, e1));
+                    tune_ms[w][c][bc] = ms / reps;
+                    if (ms < best) { best = ms; picks[w] = cand[c]; bpicks[w] = bcand[bc]; }
+                }
             g.lanes[w] = orig[w];
+            g.block[w] = borig[w];
         }
         // the sweeps' outputs as the state's own lane counts computed them (the
         // candidates sum in other orders), then the forces as the DEM left them
@@ -674,6 +687,8 @@ class Engine final : public dmph_handle {
         launch_sweep2<R>(g, a, stream);
         g.lanes[0] = picks[0];
         g.lanes[1] = picks[1];
+        g.block[0] = bpicks[0];
+        g.block[1] = bpicks[1];
         ev_pool.push_back(e0);
         ev_pool.push_back(e1);
         HCHK(hipMemcpyAsync(a.fx, save, n * sizeof(R), hipMemcpyDeviceToDevice, stream));
@@ -688,7 +703,7 @@ class Engine final : public dmph_handle {
         if (lanes) { lanes[0] = g.lanes[0]; lanes[1] = g.lanes[1]; }
         return DMPH_OK;
     }
-    float tune_ms[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
+    float tune_ms[2][4][3] = {};
 
     int step(int nsteps) override {
         HCHK(hipSetDevice(device));
@@ -916,6 +931,9 @@ class Engine final : public dmph_handle {
         out->nscenes = g.nscene;
         out->lanes[0] = std::is_same<R, double>::value ? 1 : g.lanes[0];
         out->lanes[1] = std::is_same<R, double>::value ? 1 : g.lanes[1];
+        out->block[0] = std::is_same<R, double>::value ? 256 : g.block[0];
+        out->block[1] = std::is_same<R, double>::value ? 256 : g.block[1];
+        out->cell_scale = cell_scale;
         Scalars sc;
         HCHK(hipMemcpyAsync(&sc, a.sc, sizeof sc, hipMemcpyDeviceToHost, stream));
         HCHK(hipStreamSynchronize(stream));

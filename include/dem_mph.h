@@ -75,6 +75,13 @@ typedef struct dmph_options {
     int compensated;    /* FP32 only: pair sums carried in FP64 accumulators (1, default) or FP32 (0) */
     int graph;          /* 1 (default): each step replays one captured hipGraph; 0: launches one by one */
     int lanes;          /* FP32 only: lanes per particle in the sweeps, 2, 4, 8 or 16 (0: 8; see dmph_autotune) */
+    int rigid_precision; /* FP32 only: DMPH_PREC_FP64 (default: FP64 rigid-body sums and integration, the
+                            mixed-precision solver) or DMPH_PREC_FP32 (FP32 there too: the plain FP32 solver) */
+    int block;          /* FP32 only: threads per workgroup of the pair sweeps, 64, 128 or 256 (0: 256; see dmph_autotune) */
+    double cell_scale;  /* FP32 only: cell width = cell_scale x (MaxRadius + 0.1 d), cell_scale in [1, 2]
+                           (0 or 1: the reference's cell grid, source/DEM-MPH.cpp:1211-1239); the 3^D stencil
+                           still covers the cutoff, rows hold the same pairs, slot order and list-build
+                           cost change */
 } dmph_options;
 
 void dmph_default_options(dmph_options *opt);
@@ -197,6 +204,8 @@ typedef struct dmph_info {
     int64_t rebuilds;   /* binning passes so far (device counter) */
     int nscenes;        /* scenes in the handle (dmph_create_batch), 1 otherwise; cells is per scene */
     int lanes[2];       /* FP32 sweeps 1 and 2: lanes per particle (dmph_autotune); 1 in FP64 */
+    int block[2];       /* FP32 sweeps 1 and 2: threads per workgroup (dmph_autotune); 256 in FP64 */
+    double cell_scale;  /* cell width over MaxRadius + 0.1 d (1: the reference's grid) */
 } dmph_info;
 int dmph_get_info(dmph_handle *h, dmph_info *info);
 
@@ -232,12 +241,15 @@ int dmph_set_timing(dmph_handle *h, int enable);
 int dmph_get_timing(dmph_handle *h, int group, double *total_ms, int64_t *launches);
 
 /* Autotuner (no reference counterpart): times the FP32 sweeps with 2, 4, 8
- * and 16 lanes per particle on the handle's current scene(s) -- `reps` launches
- * per candidate, re-running the sweeps on the current state (they recompute the
- * same values; forces are restored), so the simulation does not advance -- and
- * keeps the fastest for each sweep; the step graph is captured again.  The
+ * and 16 lanes per particle and workgroups of 64, 128 and 256 threads on the
+ * handle's current scene(s) -- `reps` launches per candidate, re-running the
+ * sweeps on the current state (they recompute the same values; forces are
+ * restored), so the simulation does not advance -- and keeps the fastest
+ * (lanes, workgroup) for each sweep; the step graph is captured again.  The
  * choice changes FP32 summation order only.  FP64 handles keep one thread per
- * particle.  Writes the choice to lanes[2] if not NULL. */
+ * particle.  Writes the chosen lanes to lanes[2] if not NULL; dmph_get_info
+ * reports lanes and workgroup sizes.  The cell size (dmph_options.cell_scale)
+ * needs a new handle and is tuned by dem_mph_amd.autotune. */
 int dmph_autotune(dmph_handle *h, int reps, int *lanes);
 
 const char *dmph_last_error(const dmph_handle *h);

@@ -271,13 +271,17 @@ def _rigid_scene():
     return g, sc
 
 
-@pytest.mark.parametrize("prec,tol", [(D.PREC_FP64, 1e-10), (D.PREC_FP32, 2e-5)])
-def test_rigid_bodies_match_oracle(prec, tol):
-    """Body state after 12 steps (FP64 reductions on the GPU in both precisions).
-    FP64: 1e-10 relative; FP32 pairs: 2e-5 relative on V, L and 1e-9 m on X."""
+@pytest.mark.parametrize("prec,body,tol,tolx", [(D.PREC_FP64, D.PREC_FP64, 1e-10, 1e-12),
+                                                (D.PREC_FP32, D.PREC_FP64, 2e-5, 1e-9),
+                                                (D.PREC_FP32, D.PREC_FP32, 1e-4, 1e-7)])
+def test_rigid_bodies_match_oracle(prec, body, tol, tolx):
+    """Body state after 12 steps against the oracle's FP64 rigid model.
+    FP64: 1e-10 relative on V, L, q and 1e-12 m on X.  Mixed (FP32 pairs,
+    FP64 body sums and integration): 2e-5 relative, X 1e-9 m.  Plain FP32
+    (body sums and integration in FP32 too): 1e-4 relative, X 1e-7 m."""
     g, sc = _rigid_scene()
     p = G.params(g)
-    s = D.Solver(D.Params.from_dict(p), sc, precision=prec, rigid=True)
+    s = D.Solver(D.Params.from_dict(p), sc, precision=prec, rigid=True, rigid_precision=body)
     o = P.Oracle(P.params_from_dict(p), sc, sort_mode=P.SORT_CANONICAL, rigid=True)
     s.step(12); o.step(12)
     b = s.bodies()
@@ -286,9 +290,58 @@ def test_rigid_bodies_match_oracle(prec, tol):
     q = o.get("body_q").reshape(-1, 4)
     errs = {"X": np.abs(b["x"] - X).max(), "V": np.abs(b["v"] - V).max() / np.abs(V).max(),
             "L": np.abs(b["l"] - Lm).max() / max(np.abs(Lm).max(), 1e-30), "q": np.abs(b["q"] - q).max()}
-    _report("rigid-%d" % prec, errs)
-    assert errs["X"] <= (1e-12 if prec == D.PREC_FP64 else 1e-9)
+    _report("rigid-%d-body%d" % (prec, body), errs)
+    assert errs["X"] <= tolx
     assert errs["V"] <= tol and errs["L"] <= max(tol, 1e-8) and errs["q"] <= max(tol, 1e-9)
+
+
+@pytest.mark.parametrize("name,block,cell_scale", [("mix3d", 64, 1.0), ("st3d", 128, 1.0), ("mix2d", 64, 1.0),
+                                                   ("mix3d", 256, 1.5), ("c1", 128, 1.5), ("st2d", 256, 1.25)])
+def test_fp32_workgroups_and_cell_sizes(name, block, cell_scale):
+    """The autotuner's other parameters: sweeps in workgroups of 64 or 128
+    threads, and cells wider than MaxRadius + 0.1 d (cell_scale).  Rows still
+    hold exactly the FP64 rows' pairs at the first binning (the cutoff test is
+    exact and the 3^D stencil of a wider cell still covers it), and one step
+    stays within the FP32 tolerances against the FP64 oracle on the reference's
+    own cell grid (number density 1e-5 per particle, fluid acceleration 1e-5
+    over the field, solid force 5e-5 over the field).  With wider cells the
+    solid force is not held to it: a DEM particle takes the coefficients of
+    its last contact in row order (source/DEM-MPH.cpp:2331-2396) and rows
+    follow the cell grid, so another contact's coefficients can win; the
+    force stays finite and is reported."""
+    sc, prm, L = _case(name)
+    s = D.Solver(D.Params.from_dict(prm), sc, precision=D.PREC_FP32, block=block, cell_scale=cell_scale)
+    info = s.info()
+    assert tuple(info.block) == (block, block) and abs(info.cell_scale - cell_scale) < 1e-12
+    s64 = D.Solver(D.Params.from_dict(prm), sc, precision=D.PREC_FP64)
+    cw, cw64 = np.array(info.cell_width[:]), np.array(s64.info().cell_width[:])
+    # cells at least cell_scale x (MaxRadius + 0.1 d) wide: never narrower than the reference's
+    assert np.all(cw[:sc.dim] >= cell_scale * (info.max_radius + 0.1 * sc.d) * (1 - 1e-12))
+    assert np.all(cw[:sc.dim] >= cw64[:sc.dim] * (1 - 1e-12))
+    p32, i32 = s.pairs()
+    p64, i64 = s64.pairs()
+    id32, id64 = s.field("id", D.ORDER_SLOT), s64.field("id", D.ORDER_SLOT)
+    # rows as sets of original ids, keyed by original id
+    r32 = {int(id32[k]): frozenset(id32[i32[p32[k]:p32[k + 1]]].tolist()) for k in range(len(id32))}
+    r64 = {int(id64[k]): frozenset(id64[i64[p64[k]:p64[k + 1]]].tolist()) for k in range(len(id64))}
+    assert r32 == r64
+    s64.close()
+    o = P.Oracle(P.params_from_dict(prm), sc, sort_mode=P.SORT_CANONICAL)
+    s.step(1); o.step(1)
+    n0p = o.scalars()["n0p"]
+    nd, ndr = s.field("vol_strain_p") + n0p, o.field("vol_strain_p") + n0p
+    typ = o.field("type")
+    F, Fr, m = s.field("force"), o.field("force"), o.field("mass")
+    fl, so = typ < 2, (typ >= 2) & (typ < 4)
+    errs = {"density_per_particle": np.max(np.abs(nd - ndr) / np.abs(ndr)),
+            "fluid_accel": np.linalg.norm((F[fl] - Fr[fl]) / m[fl, None]) / np.linalg.norm(Fr[fl] / m[fl, None]),
+            "solid_force": np.linalg.norm(F[so] - Fr[so]) / np.linalg.norm(Fr[so]) if so.any() else 0.0}
+    _report("%s-block%d-cells%.2f" % (name, block, cell_scale), errs)
+    assert errs["density_per_particle"] <= 1e-5 and errs["fluid_accel"] <= 1e-5
+    assert np.isfinite(F).all()
+    if cell_scale == 1.0:
+        assert errs["solid_force"] <= 5e-5
+    s.close(); o.close()
 
 
 @pytest.mark.parametrize("prec", [D.PREC_FP64, D.PREC_FP32])
@@ -469,6 +522,7 @@ def test_autotune_keeps_state_and_tolerance():
     t0 = s.time()
     lanes = s.autotune(3)
     assert set(lanes) <= {2, 4, 8, 16} and tuple(s.info().lanes) == lanes
+    assert set(s.info().block) <= {64, 128, 256}
     assert s.time() == t0
     for f in fields:
         np.testing.assert_array_equal(s.field(f), before[f], err_msg=f)
