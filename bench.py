@@ -336,15 +336,20 @@ def channel_bench(D, steps: int, warmup: int, seed: int, surrogate: bool):
     return out
 
 
-def surrogate_bench(D, scenes, seq_ms_per_dt: float, batched_ms_per_dt: float, chunk: int, steps: int,
-                    eval_scenes: int, eval_steps: int, seed: int):
-    """The learned surrogate on the ensemble: throughput of surrogate + fallback
-    (one plain-solver stride to re-anchor every 200 surrogate steps, and the
-    fallen-back share of the scenes advanced by the solver, both charged at the
-    batched solver's rate) on all scenes in chunks, and accuracy on held-out
-    scenes against the FP64 solver (the GPU parity mode, equal to the FP64
-    oracle to ~1e-9 per step) next to the two baselines without a network and
-    the mixed-precision solver's own error."""
+def surrogate_bench(D, scenes, seq_ms_per_dt: float, batched_plain_ms_per_dt: float, chunk: int, steps: int,
+                    eval_scenes: int, eval_steps: int, hybrid_strides: int, seed: int):
+    """The learned surrogate on the ensemble.
+    - hybrid: the surrogate with its plain-solver fallback as a deployment runs
+      it (dem_mph_amd.surrogate.hybrid: solver warm-up, re-anchoring and a
+      probe-based judge every 200 strides, fallen-back scenes on the solver),
+      timed end to end from t = 0 on `eval_scenes` held-out scenes for
+      `hybrid_strides` strides, against the plain FP32 solver batch advancing
+      the same scenes over the same simulated time; accuracy against the
+      mixed-precision solver's trajectory.
+    - rollout_512: the surrogate alone on all scenes (in chunks), per stride.
+    - accuracy_vs_baselines: 100-stride rollouts of the held-out scenes re-
+      anchored on the FP64 solver's trajectory (an evaluation of the network,
+      not a deployment), next to rollouts without a network."""
     from dem_mph_amd import scenes as S
     from dem_mph_amd import surrogate as SG
     if not os.path.exists(SURROGATE_WEIGHTS):
@@ -352,6 +357,18 @@ def surrogate_bench(D, scenes, seq_ms_per_dt: float, batched_ms_per_dt: float, c
     sur = SG.load(SURROGATE_WEIGHTS)
     params = S.params_c3()
     dt = params["Dt"]
+    ev = S.ensemble_c3(eval_scenes, seed=seed + 1000)
+    print("bench: surrogate + fallback on %d held-out scenes, %d strides" % (len(ev), hybrid_strides),
+          file=sys.stderr, flush=True)
+    hyb = SG.hybrid_report(sur, ev, params, hybrid_strides)
+    hyb.pop("judgements_detail", None)
+    j = hyb.pop("judgements")
+    hyb["judge"] = {"checks": len(j), "fell_back_checks": sum(1 for x in j if x["fell_back"]),
+                    "invalid_states": hyb.pop("invalid_states", 0),
+                    "true_error_m_mean": float(np.mean([x["true_error_m"] for x in j])) if j else None,
+                    "estimate_m_mean": float(np.mean([x["estimate_m"] for x in j])) if j else None,
+                    "kept_with_true_error_over_tol": sum(1 for x in j if not x["fell_back"] and x["true_error_m"] > 0.008),
+                    "fell_with_true_error_within_tol": sum(1 for x in j if x["fell_back"] and x["true_error_m"] <= 0.008)}
     t_step = 0.0
     finite = True
     nch = (len(scenes) + chunk - 1) // chunk   # equal chunks of at most `chunk` scenes
@@ -363,25 +380,13 @@ def surrogate_bench(D, scenes, seq_ms_per_dt: float, batched_ms_per_dt: float, c
         print("bench: surrogate chunk %d: %.1f ms/step for %d nodes" % (k, r["seconds_per_step"] * 1e3, r["nodes"]),
               file=sys.stderr, flush=True)
     n = sum(sc.n for sc in scenes)
-    reanchor = 200
-    ev = S.ensemble_c3(eval_scenes, seed=seed + 1000)
-    acc = SG.evaluate(sur, ev, params, 2000, eval_steps, reanchor, 0.008, fp64=True)
-    # charged per surrogate step: the surrogate on every scene, one solver stride per
-    # `reanchor` steps to re-anchor, and the scenes that fell back in the held-out
-    # evaluation (their share) advanced by the batched solver as well (an upper bound:
-    # they are not taken out of the surrogate's batch)
-    fell = acc["surrogate"]["fell_back"] / max(len(ev), 1)
-    t_anchor = (SG.STRIDE * batched_ms_per_dt * 1e-3) / reanchor
-    t_fallback = fell * SG.STRIDE * batched_ms_per_dt * 1e-3
-    t_total = t_step + t_anchor + t_fallback
-    out = {"surrogate_ms_per_step": t_step * 1e3, "reanchor_ms_per_step": t_anchor * 1e3,
-           "fallback_ms_per_step": t_fallback * 1e3, "fallback_fraction": fell, "stride_dt": SG.STRIDE,
-           "sim_s_per_wall_s": SG.STRIDE * dt / t_total,
-           "particle_steps_per_s": n * SG.STRIDE / t_total, "finite": finite,
-           "vs_sequential_plain": (SG.STRIDE * dt / t_total) / (dt / (seq_ms_per_dt * 1e-3)),
-           "vs_batched_plain": (SG.STRIDE * dt / t_total) / (dt / (batched_ms_per_dt * 1e-3)),
-           "accuracy": acc}
-    return out
+    roll = {"scenes": len(scenes), "surrogate_ms_per_stride": t_step * 1e3, "stride_dt": SG.STRIDE,
+            "sim_s_per_wall_s": SG.STRIDE * dt / t_step, "particle_steps_per_s": n * SG.STRIDE / t_step,
+            "finite": finite,
+            "vs_sequential_plain": (SG.STRIDE * dt / t_step) / (dt / (seq_ms_per_dt * 1e-3)),
+            "vs_batched_plain": (SG.STRIDE * dt / t_step) / (dt / (batched_plain_ms_per_dt * 1e-3))}
+    acc = SG.evaluate(sur, ev, params, 2000, eval_steps, 200, 0.008, fp64=True)
+    return {"hybrid": hyb, "rollout_512": roll, "accuracy_vs_baselines": acc}
 
 
 def main(argv=None) -> int:
@@ -406,6 +411,8 @@ def main(argv=None) -> int:
     ap.add_argument("--surrogate-chunk", type=int, default=96,
                     help="scenes per surrogate batch (96: ~7.8M nodes, within the 4 GB row offsets of the edge gathers)")
     ap.add_argument("--surrogate-eval-scenes", type=int, default=64)
+    ap.add_argument("--hybrid-strides", type=int, default=400,
+                    help="surrogate strides of the timed surrogate + fallback run (two re-anchor periods)")
     ap.add_argument("--no-channel", action="store_true", help="skip the large-channel (C5) measurement")
     ap.add_argument("--channel-steps", type=int, default=100)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -495,21 +502,38 @@ def main(argv=None) -> int:
         o.close()
     else:
         out["cpu_baseline"] = None
+    # the secondary workloads: a failure is recorded in its own entry (with the
+    # error) and the headline line is still printed; the process exits non-zero
+    failed = []
+
+    def section(name, fn):
+        try:
+            return fn()
+        except Exception as e:   # noqa: BLE001 -- recorded, and the exit status reports it
+            import traceback
+            traceback.print_exc()
+            failed.append(name)
+            return {"error": "%s: %s" % (type(e).__name__, e)}
+
     if not a.no_ensemble:
         print("bench: batched ensemble", file=sys.stderr, flush=True)
-        ens, scenes = ensemble_bench(D, a.ensemble_scenes, a.ensemble_steps, 5, a.seed, a.ensemble_sample)
+        res = section("ensemble", lambda: ensemble_bench(D, a.ensemble_scenes, a.ensemble_steps, 5, a.seed,
+                                                         a.ensemble_sample))
+        ens, scenes = res if isinstance(res, tuple) else (res, None)
         out["ensemble"] = ens
-        if not a.no_surrogate:
+        if not a.no_surrogate and scenes is not None:
             print("bench: learned surrogate", file=sys.stderr, flush=True)
-            ens["surrogate_plus_fallback"] = surrogate_bench(
+            ens["surrogate"] = section("surrogate", lambda: surrogate_bench(
                 D, scenes, ens["sequential_plain"]["ms_per_step"], ens["batched_plain"]["ms_per_step"],
-                a.surrogate_chunk, 3, a.surrogate_eval_scenes, 100, a.seed)
+                a.surrogate_chunk, 3, a.surrogate_eval_scenes, 100, a.hybrid_strides, a.seed))
         del scenes
     if not a.no_channel:
         print("bench: large channel", file=sys.stderr, flush=True)
-        out["channel"] = channel_bench(D, a.channel_steps, 5, a.seed, not a.no_surrogate)
+        out["channel"] = section("channel", lambda: channel_bench(D, a.channel_steps, 5, a.seed, not a.no_surrogate))
+    if failed:
+        out["failed_sections"] = failed
     print(json.dumps(out))
-    return 0
+    return 1 if failed else 0
 
 
 if __name__ == "__main__":

@@ -313,7 +313,7 @@ class Engine final : public dmph_handle {
             err = "more than 67,108,863 particles";
             return DMPH_E_ARG;
         }
-        {   // neighbour-list capacity: 1.6 x the lattice count within MaxRadius + 0.1 d, in
+        {   // neighbour-list capacity: 2 x the lattice count within MaxRadius + 0.1 d, in
             // multiples of 16; FP32 rows split it into a near and a far segment, each
             // sized from its own lattice count
             // near segment (FP32): every pass but the viscosity and surface-tension
@@ -331,11 +331,16 @@ class Engine final : public dmph_handle {
                         cnn += q <= std::min(rc, rnd) * std::min(rc, rnd);
                     }
             auto up16 = [](double x) { return ((int)ceil(x) + 15) / 16 * 16; };
+            // headroom over a lattice: the dam-break fluid piles up against the
+            // walls (rows of 142 entries, far segments of 80+, measured over
+            // 0.2 s of the C3 scenes); capacity costs memory only, the sweeps
+            // read the entries a row holds
+            const double kFill = 2.0;
             if (std::is_same<R, double>::value) {
-                g.kcap = g.knear = up16(1.6 * cnt);
+                g.kcap = g.knear = up16(kFill * cnt);
             } else {
-                g.knear = up16(1.6 * cnn);
-                g.kcap = g.knear + up16(1.6 * (cnt - cnn));
+                g.knear = up16(kFill * cnn);
+                g.kcap = g.knear + up16(kFill * (cnt - cnn));
             }
             if (getenv("DMPH_NEIGHBOR_CAPACITY")) {
                 const int total = std::max(32, up16(atoi(getenv("DMPH_NEIGHBOR_CAPACITY"))));

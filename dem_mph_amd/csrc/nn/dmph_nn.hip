@@ -701,7 +701,11 @@ __global__ void __launch_bounds__(256) k_edge_fwd_h(const h2_t *__restrict__ AB,
     // B_j pair of this lane
     auto b_of = [&](int j) {
         if constexpr (BUF) {
-            return bitsh2(__builtin_amdgcn_raw_buffer_load_b32(rsrc, (int)lane_bytes, j * 512, 0));
+            // the row offset in unsigned arithmetic: up to 4 GB (N * 512 < 2^32 is checked
+            // by the launcher), past the int range for j >= 2^22; the bits are what the
+            // scalar offset register takes
+            return bitsh2(__builtin_amdgcn_raw_buffer_load_b32(rsrc, (int)lane_bytes,
+                                                               (int)((uint32_t)j << 9), 0));
         } else {
             const char *row = reinterpret_cast<const char *>(AB) + (size_t)j * 512 + 256;
             return *reinterpret_cast<const h2_t *>(row + lane_bytes);

@@ -642,6 +642,7 @@ class Trajectories:
                 s.step(st - cur)
                 cur = st
             s.export_state(self.pos[f].data_ptr(), self.vel[f].data_ptr())
+        s.sync()   # raises if a neighbour row overflowed its capacity: the frames would be invalid
         self.sim_seconds = _now() - t0
         self.solver_steps = cur
         s.close()
@@ -827,6 +828,326 @@ def advance_batch(sur: Surrogate, scenes, steps: int, dt: float) -> Dict[str, fl
     el = _now() - t0
     ok = bool(torch.isfinite(pos).all().item())
     return {"seconds_per_step": el / steps, "nodes": n, "finite": ok}
+
+
+class _Active:
+    """The surrogate's batch: a subset of an ensemble's scenes as node arrays on
+    the GPU (class, body ids renumbered from 0 over the subset, scene ids
+    0..m-1) and the subset's particle offsets."""
+
+    def __init__(self, torch, dev, scenes, ids):
+        self.ids = list(ids)
+        sub = [scenes[k] for k in self.ids]
+        self.m = len(sub)
+        self.off = np.cumsum([0] + [sc.n for sc in sub]).astype(np.int64)
+        n = int(self.off[-1])
+        typ = np.concatenate([sc.type for sc in sub]) if sub else np.zeros(0, np.int32)
+        self.cls_np = np.where(typ < 2, 0, np.where(typ < 4, 1, 2)).astype(np.int32)
+        body, boff = [], 0
+        for sc in sub:
+            b = sc.body if sc.body is not None else np.full(sc.n, -1, np.int32)
+            body.append(np.where(b >= 0, b + boff, -1).astype(np.int32))
+            boff += int(b.max()) + 1 if (b >= 0).any() else 0
+        self.body_np = np.concatenate(body) if sub else np.zeros(0, np.int32)
+        self.nbodies = boff
+        self.cls = torch.tensor(self.cls_np, device=dev)
+        self.body = torch.tensor(self.body_np, device=dev)
+        self.scene = torch.tensor(np.repeat(np.arange(self.m, dtype=np.int32), np.diff(self.off)), device=dev)
+        self.body_sums = torch.zeros((max(boff, 1), 4), dtype=torch.float32, device=dev)
+        self.n = n
+
+    def rows(self, k: int) -> slice:
+        q = self.ids.index(k)
+        return slice(int(self.off[q]), int(self.off[q + 1]))
+
+
+def _with_state(sc, pos: np.ndarray, vel: np.ndarray, time: float):
+    """A copy of scene `sc` (types, rigid bodies, densities, domain) at another state."""
+    import copy
+    out = copy.copy(sc)
+    out.pos = np.ascontiguousarray(pos, np.float64)
+    out.vel = np.ascontiguousarray(vel, np.float64)
+    out.time = time
+    return out
+
+
+def debris_centroids(sc, pos: np.ndarray) -> np.ndarray:
+    """Centroids [bodies, 3] of a scene's rigid debris bodies at positions `pos`."""
+    if sc.body is None or not (sc.body >= 0).any():
+        return np.zeros((0, 3))
+    m = sc.body >= 0
+    nb = int(sc.body.max()) + 1
+    c = np.zeros((nb, 3))
+    np.add.at(c, sc.body[m], pos[m])
+    return c / np.maximum(np.bincount(sc.body[m], minlength=nb), 1)[:, None]
+
+
+def _rms(a: np.ndarray, b: np.ndarray) -> float:
+    return float(np.sqrt(((a - b) ** 2).sum(axis=1).mean())) if len(a) else 0.0
+
+
+def hybrid(sur: Surrogate, scenes, params, steps: int, reanchor: int = 200, probe: int = HIST - 1,
+           tol: float = 0.008) -> Dict[str, object]:
+    """The surrogate with its plain-solver fallback as a deployment runs it,
+    from t = 0, on one GPU, for `steps` strides (each STRIDE dt) after the
+    warm-up, the probes' strides included:
+
+    1. the plain FP32 solver advances every scene (HIST - 1) strides, writing
+       the velocity history the network reads;
+    2. the surrogate advances the scenes it holds; every `reanchor` strides
+       it is re-anchored and judged: a solver batch is built from the
+       surrogate's own state (no other trajectory exists) and advanced `probe`
+       strides while the surrogate, from the same state, predicts the same
+       strides; the debris-centroid RMS difference after them, extrapolated
+       linearly to a re-anchor period (x reanchor / probe), is the judge's
+       estimate of the error the surrogate would add before the next check;
+    3. a scene whose estimate exceeds `tol`, or whose state the solver cannot
+       take (a neighbour row overflows: packed denser than the physics allows),
+       falls back: the solver recomputes it from its last anchored solver state
+       (the surrogate's strides since are discarded), it joins the fallback
+       solver batch and stays there to the end; the others resume from the
+       solver's state after the probe (positions and the `probe` + 1 frames of
+       history), which becomes their new anchor;
+    4. the fallback batch advances STRIDE solver steps per surrogate stride.
+
+    Every part is inside the clock (set-up of the solver batches included).
+    Returns the wall time, the simulated time, per-scene final states (host,
+    FP64) and the judge's records."""
+    import torch
+    import dem_mph_amd as D
+    dev = sur.dev
+    dt = params["Dt"]
+    h = STRIDE * dt
+    plain = dict(precision=D.PREC_FP32, compensated=False, rigid_precision=D.PREC_FP32, rigid=True)
+    prm = D.Params.from_dict(params)
+    sync = lambda: torch.cuda.synchronize(dev)
+    t_start = _now()
+    act = _Active(torch, dev, scenes, range(len(scenes)))
+    N = act.n
+    # 1. warm-up on the solver: HIST frames STRIDE steps apart
+    fp = torch.empty((HIST, N, 3), dtype=torch.float32, device=dev)
+    fv = torch.empty((HIST, N, 3), dtype=torch.float32, device=dev)
+    s = D.Solver(prm, scenes, **plain)
+    for f in range(HIST):
+        if f:
+            s.step(STRIDE)
+        s.export_state(fp[f].data_ptr(), fv[f].data_ptr())
+    s.sync()
+    s.close()
+    pos = fp[HIST - 1].clone()
+    vh = torch.stack([fv[HIST - 1 - t] for t in range(HIST)]).contiguous()
+    done = HIST - 1                       # strides simulated
+    # the last anchored (solver) state of each scene: stride and host copy
+    P0, V0 = fp[HIST - 1].cpu().numpy().astype(np.float64), fv[HIST - 1].cpu().numpy().astype(np.float64)
+    anchor = {k: (done, (P0[act.rows(k)], V0[act.rows(k)])) for k in act.ids}
+    del fp, fv, P0, V0
+    fb, fb_ids = None, []
+    judged = []
+    t_solver = t_sur = 0.0
+
+    def advance_fallback(nstrides):
+        nonlocal t_solver
+        if fb is not None and nstrides > 0:
+            t0 = _now()
+            fb.step(STRIDE * nstrides)
+            t_solver += _now() - t0
+
+    def host_state(k, P, V, rows):
+        return P[rows].astype(np.float64), V[rows].astype(np.float64)
+
+    def rebuild_fallback(new_states):
+        """fallback batch = its scenes at their current state + `new_states` {scene: (pos, vel)}"""
+        nonlocal fb, fb_ids
+        states = dict(new_states)
+        if fb is not None:
+            fb.sync()
+            P, V = fb.field("pos"), fb.field("vel")
+            for q, k in enumerate(fb_ids):
+                sl = slice(int(fb.scene_offsets[q]), int(fb.scene_offsets[q + 1]))
+                states[k] = (P[sl], V[sl])
+            fb.close()
+        fb_ids = sorted(states)
+        fb = D.Solver(prm, [_with_state(scenes[k], *states[k], done * h) for k in fb_ids], **plain)
+
+    while done < (HIST - 1) + steps:
+        left = (HIST - 1) + steps - done
+        n = min(reanchor, left)
+        t0 = _now()
+        for _ in range(n):
+            if act.m:
+                y = sur.infer(pos, vh, act.cls, act.scene, act.m)
+                sur.rollout_step(pos, vh, y, act.cls, act.body, act.nbodies, h, act.body_sums)
+            if fb is not None:
+                fb.step(STRIDE)
+        sync()
+        t_sur += _now() - t0
+        done += n
+        if done >= (HIST - 1) + steps or act.m == 0:
+            continue
+        # 2. re-anchor and judge the surrogate's scenes
+        P = pos.cpu().numpy()
+        V = vh[0].cpu().numpy()
+        finite = {k: bool(np.isfinite(P[act.rows(k)]).all() and np.isfinite(V[act.rows(k)]).all()) for k in act.ids}
+        probe_ids = [k for k in act.ids if finite[k]]
+        lost = [k for k in act.ids if not finite[k]]
+        est, fall = {}, set(lost)
+        frames = {}   # scene -> (positions, velocities) [probe + 1, n_k, 3] of the solver's probe
+        if probe_ids:
+            t0 = _now()
+            p2, v2 = pos.clone(), vh.clone()
+            for _ in range(probe):
+                y = sur.infer(p2, v2, act.cls, act.scene, act.m)
+                sur.rollout_step(p2, v2, y, act.cls, act.body, act.nbodies, h, act.body_sums)
+            sync()
+            P2 = p2.cpu().numpy()
+            del p2, v2
+            t_sur += _now() - t0
+            t0 = _now()
+
+            def run_probe(ids):
+                ps = D.Solver(prm, [_with_state(scenes[k], *host_state(k, P, V, act.rows(k)), done * h) for k in ids],
+                              **plain)
+                try:
+                    fpp = torch.empty((probe + 1, ps.n, 3), dtype=torch.float32, device=dev)
+                    fvv = torch.empty((probe + 1, ps.n, 3), dtype=torch.float32, device=dev)
+                    ps.export_state(fpp[0].data_ptr(), fvv[0].data_ptr())
+                    for f in range(1, probe + 1):
+                        ps.step(STRIDE)
+                        ps.export_state(fpp[f].data_ptr(), fvv[f].data_ptr())
+                    ps.sync()   # raises when a row overflowed: a state too dense to be physical
+                    off_ = ps.scene_offsets
+                    return {k: (fpp[:, int(off_[q]):int(off_[q + 1])], fvv[:, int(off_[q]):int(off_[q + 1])])
+                            for q, k in enumerate(ids)}
+                finally:
+                    ps.close()
+
+            try:
+                frames = run_probe(probe_ids)
+            except D.DmphError:
+                for k in probe_ids:   # which scenes: one at a time
+                    try:
+                        frames.update(run_probe([k]))
+                    except D.DmphError:
+                        lost.append(k)
+                        fall.add(k)
+            t_solver += _now() - t0
+            for k, (fpp, _) in frames.items():
+                c_sol = debris_centroids(scenes[k], fpp[probe].cpu().numpy().astype(np.float64))
+                c_sur = debris_centroids(scenes[k], P2[act.rows(k)].astype(np.float64))
+                est[k] = _rms(c_sur, c_sol) * reanchor / probe
+                if est[k] > tol:
+                    fall.add(k)
+        judged.append({"stride": done, "estimate_m": {int(k): v for k, v in est.items()},
+                       "centroids": {int(k): debris_centroids(scenes[k], P[act.rows(k)].astype(np.float64)).tolist()
+                                     for k in frames},
+                       "fell_back": sorted(int(k) for k in fall), "invalid_state": sorted(int(k) for k in lost)})
+        advance_fallback(probe)   # the fallback batch keeps pace with the probe
+        done += probe
+        t0 = _now()
+        if fall:
+            # 3. the solver takes over: each fallen scene is recomputed from its last
+            # anchored solver state (the surrogate's strides since are discarded);
+            # every scene on the surrogate was anchored at the same stride
+            a_done = {anchor[k][0] for k in fall}
+            assert len(a_done) == 1
+            a_done = a_done.pop()
+            ids = sorted(fall)
+            cs = D.Solver(prm, [_with_state(scenes[k], *anchor[k][1], a_done * h) for k in ids], **plain)
+            cs.step(STRIDE * (done - a_done))
+            cs.sync()
+            Pc, Vc = cs.field("pos"), cs.field("vel")
+            newstates = {k: (Pc[int(cs.scene_offsets[q]):int(cs.scene_offsets[q + 1])],
+                             Vc[int(cs.scene_offsets[q]):int(cs.scene_offsets[q + 1])]) for q, k in enumerate(ids)}
+            cs.close()
+            rebuild_fallback(newstates)
+        # the surrogate resumes from the solver's state for the scenes it keeps
+        keep = [k for k in act.ids if k in frames and k not in fall]
+        if keep:
+            pos = torch.cat([frames[k][0][probe] for k in keep]).contiguous()
+            vh = torch.stack([torch.cat([frames[k][1][probe - t] for k in keep]) for t in range(HIST)]).contiguous()
+            for k in keep:
+                anchor[k] = (done, (frames[k][0][probe].cpu().numpy().astype(np.float64),
+                                    frames[k][1][probe].cpu().numpy().astype(np.float64)))
+        act = _Active(torch, dev, scenes, keep)
+        if not keep:
+            pos = torch.empty((0, 3), dtype=torch.float32, device=dev)
+        t_solver += _now() - t0
+    sync()
+    if fb is not None:
+        fb.sync()
+    elapsed = _now() - t_start
+    # final states, host FP64, per scene
+    final = {}
+    if act.m:
+        P = pos.cpu().numpy().astype(np.float64)
+        for k in act.ids:
+            final[k] = P[act.rows(k)]
+    if fb is not None:
+        P = fb.field("pos")
+        for q, k in enumerate(fb_ids):
+            final[k] = P[int(fb.scene_offsets[q]):int(fb.scene_offsets[q + 1])]
+        fb.close()
+    return {"seconds": elapsed, "surrogate_seconds": t_sur, "solver_seconds": t_solver,
+            "strides": done, "solver_steps_equivalent": done * STRIDE, "simulated_s": done * h,
+            "on_surrogate_at_end": len(act.ids), "fell_back": len(fb_ids), "judged": judged, "final": final}
+
+
+def hybrid_report(sur: Surrogate, scenes, params, steps: int, reanchor: int = 200, tol: float = 0.008,
+                  reference_precision: Optional[int] = None) -> Dict[str, object]:
+    """hybrid() timed against the plain FP32 solver batch advancing the same
+    scenes over the same simulated time from t = 0 (autotuned, outside the
+    clock), and its accuracy against a reference solver trajectory (the
+    mixed-precision solver unless `reference_precision` is FP64): debris-
+    centroid RMS per scene at the end and, at each judgement, the true error of
+    the surrogate's state next to the judge's estimate."""
+    import torch
+    import dem_mph_amd as D
+    r = hybrid(sur, scenes, params, steps, reanchor=reanchor, tol=tol)
+    total = r["solver_steps_equivalent"]
+    dt = params["Dt"]
+    prm = D.Params.from_dict(params)
+    s = D.Solver(prm, scenes, precision=D.PREC_FP32, compensated=False, rigid_precision=D.PREC_FP32, rigid=True)
+    s.autotune(2)   # on the initial state, outside the clock
+    s.sync()
+    t0 = _now()
+    s.step(total)
+    s.sync()
+    t_plain = _now() - t0
+    s.close()
+    # reference: frames at every judgement and at the end
+    marks = [j["stride"] * STRIDE for j in r["judged"]] + [total]
+    prec = D.PREC_FP32 if reference_precision is None else reference_precision
+    ref = Trajectories(scenes, params, marks, precision=prec)
+    off = ref.off
+    Pend = ref.pos[len(marks) - 1].cpu().numpy().astype(np.float64)
+    err = np.zeros(len(scenes))
+    for k, sc in enumerate(scenes):
+        err[k] = _rms(debris_centroids(sc, r["final"][k]), debris_centroids(sc, Pend[int(off[k]):int(off[k + 1])]))
+    judge = []
+    for f, j in enumerate(r["judged"]):
+        Pf = ref.pos[f].cpu().numpy().astype(np.float64)
+        for k, c in j["centroids"].items():
+            true = _rms(np.asarray(c), debris_centroids(scenes[k], Pf[int(off[k]):int(off[k + 1])]))
+            judge.append({"stride": j["stride"], "scene": k, "true_error_m": true, "estimate_m": j["estimate_m"][k],
+                          "fell_back": k in j["fell_back"]})
+    del ref
+    sim = r["simulated_s"]
+    fb_set = set(k for j in r["judged"] for k in j["fell_back"])
+    return {"scenes": len(scenes), "particles": int(sum(sc.n for sc in scenes)), "surrogate_strides": steps,
+            "stride_dt": STRIDE, "reanchor_strides": reanchor, "probe_strides": HIST - 1, "tolerance_m": tol,
+            "simulated_s": sim, "seconds": r["seconds"], "surrogate_seconds": r["surrogate_seconds"],
+            "solver_seconds": r["solver_seconds"], "sim_s_per_wall_s": sim / r["seconds"],
+            "plain_batched_seconds": t_plain, "plain_batched_sim_s_per_wall_s": sim / t_plain,
+            "vs_plain_batched": t_plain / r["seconds"],
+            "fell_back": r["fell_back"], "on_surrogate_at_end": r["on_surrogate_at_end"],
+            "invalid_states": sum(len(j["invalid_state"]) for j in r["judged"]),
+            "reference": "mixed-precision solver" if prec == D.PREC_FP32 else "FP64 solver",
+            "debris_centroid_rms_m_mean": float(err.mean()), "debris_centroid_rms_m_max": float(err.max()),
+            "within_tolerance": int((err <= tol).sum()),
+            "within_tolerance_of_surrogate_kept": int(sum(1 for k in range(len(scenes))
+                                                         if k not in fb_set and err[k] <= tol)),
+            "surrogate_kept": int(len(scenes) - len(fb_set)),
+            "judgements": judge}
 
 
 def save(sur: Surrogate, path: str, **meta) -> None:

@@ -571,3 +571,57 @@ def test_training_reduces_loss(S):
         sur.adam(1e-3)
     assert np.isfinite(losses).all()
     assert losses[-1] < 0.1 * losses[0], (losses[0], losses[-1])
+
+
+def test_half_edge_sums_rows_past_2_22(S):
+    """The inference edge sums' buffer-descriptor path (taken while N x 512
+    bytes of [A | B] rows fit 4 GB) with rows that reference nodes j >= 2^22,
+    whose row offset j x 512 no longer fits a signed int: N = 5,000,000 nodes,
+    128 rows (half of them at i >= 2^22) of 40 partners each, drawn from both
+    sides of 2^22; the sums against FP64 sums of the same half inputs."""
+    n = 5_000_000
+    assert n * 512 <= 0xFFFFFFFF - 1024 and n > (1 << 22)
+    g = torch.Generator(device="cuda:0")
+    g.manual_seed(5)
+    abd = (torch.randn((n, 256), generator=g, device="cuda:0", dtype=torch.float32) * 0.7).to(torch.float16)
+    pos = torch.rand((n, 3), generator=g, device="cuda:0", dtype=torch.float32) * 0.05
+    rng = np.random.default_rng(5)
+    rows = np.concatenate([rng.choice(1 << 22, 64, replace=False), (1 << 22) + rng.choice(n - (1 << 22), 64, replace=False)])
+    lens = np.zeros(n, np.int64)
+    lens[rows] = 40
+    RP = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    CI = np.zeros(int(RP[-1]), np.int32)
+    for i in rows:
+        CI[RP[i]:RP[i + 1]] = np.sort(np.concatenate([rng.integers(0, 1 << 22, 10), rng.integers(1 << 22, n, 30)]))
+    rp, ci = torch.tensor(RP, device="cuda:0"), torch.tensor(CI, device="cuda:0")
+    Wc = (rng.standard_normal((128, 4)) * 0.8).astype(np.float32)
+    bc = (rng.standard_normal(128) * 0.3).astype(np.float32)
+    Wcd, bcd = torch.tensor(Wc, device="cuda:0"), torch.tensor(bc, device="cuda:0")
+    lib = S.nnlib()
+    s = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    radius = 0.03
+    eps = torch.empty(2 * ci.numel(), dtype=torch.int32, device="cuda:0")
+    out = torch.zeros((n, 128), dtype=torch.int16, device="cuda:0")
+    assert lib.dmph_nn_edge_eps(_p(pos), _p(rp), _p(ci), _p(eps), n, C.c_float(radius), s) == 0
+    assert lib.dmph_nn_edge_fwd_f16(_p(abd), _p(rp), _p(ci), _p(eps), _p(Wcd), _p(bcd), _p(out), 128, n,
+                                    None, 0, None, None, s) == 0
+    torch.cuda.synchronize()
+    need = np.unique(np.concatenate([rows, CI]))   # the nodes in use, compacted
+    idx = torch.tensor(need, device="cuda:0")
+    AB = abd[idx].cpu().numpy()
+    P = pos[idx].cpu().numpy().astype(np.float64)
+    at = lambda v: np.searchsorted(need, v)
+    got = _bf16_float(out[torch.tensor(rows, device="cuda:0")].cpu().numpy().view(np.uint16))
+    ref = np.zeros((len(rows), 128))
+    for k, i in enumerate(rows):
+        J = at(CI[RP[i]:RP[i + 1]])
+        d = P[J] - P[at(i)]
+        e = np.concatenate([d, np.linalg.norm(d, axis=1, keepdims=True)], axis=1) / radius
+        # the kernel reads the edge features as halves: round them the same way
+        e = e.astype(np.float16).astype(np.float64)
+        A, B = AB[at(i), :128].astype(np.float64), AB[J, 128:].astype(np.float64)
+        ref[k] = np.maximum(A + bc + B + e @ Wc.T.astype(np.float64), 0.0).sum(axis=0)
+    assert int(out.count_nonzero()) > 0
+    err = np.abs(got - ref)
+    assert err.max() <= 1e-2 * np.abs(ref).max(), err.max()
+    assert np.linalg.norm(got - ref) <= 4e-3 * np.linalg.norm(ref)
