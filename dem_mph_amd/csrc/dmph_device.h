@@ -46,6 +46,7 @@ struct Geo {
     int nloose;         // solid particles outside rigid bodies (moved by k_move_solids)
     int ccap;           // DEM contact-list capacity per particle
     int body_f32;       // FP32 only: rigid-body sums and integration in FP32 (the plain solver)
+    int body_tpb;       // threads per block of the body update: the largest body's members, 64 to 1024
     double cw[3], dmin[3], dw[3];
     double d, vol, dt, dem_dt;
     double ra, rg, rp, rv, rmax, cut2, skin2;
@@ -68,6 +69,8 @@ struct Scalars {
     int list_overflow;              // longest row that did not fit the neighbour list (0 = none)
     double wall_center[2][3];
     unsigned int ticket[4];         // blocks finished, per kernel that ends in a last-block phase
+    int full_sweep;                 // FP32: > 0 -> sweep 1 covers every wall particle (k_decide counts it down)
+    int walls_moving;               // this step moved the walls (0.2 <= Time <= 0.4, written by k_predict)
 };
 
 template <class R>
@@ -101,6 +104,7 @@ struct Dev {
     int *b_moff, *b_mslot;         // per body: offsets into b_mslot [nb + 1]; member slots [members], by rank
     double *blk_max;               // per-block maxima of the squared displacement
     float4 *p4, *v4;               // FP32 sweeps: packed (position, type) and (velocity, PressureP)
+    int *wact;                     // FP32, per wall slot: its row at the last rebinning holds a fluid or solid particle
     const double *x64;             // FP32, first binning only: the input positions [n][3] in original order
     Scalars *sc;
 };

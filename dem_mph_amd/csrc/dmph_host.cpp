@@ -361,6 +361,7 @@ class Engine final : public dmph_handle {
         Scalars sc{};
         sc.time = gr->time;
         sc.flag = 1;
+        sc.full_sweep = 2;   // the first step's sweep 1 covers every particle
         sc.rebuilds = 1;
         for (int w = 0; w < 2; w++)
             for (int k = 0; k < 3; k++) sc.wall_center[w][k] = p.wall_center[w + 4][k];
@@ -405,7 +406,10 @@ class Engine final : public dmph_handle {
         TRY(alloc(a.t_dbl, g.rigid ? 3 * n : 1)); TRY(alloc(a.r0, g.rigid ? 3 * n : 1));
         TRY(alloc(a.sc, 1));
         TRY(alloc(a.blk_max, (size_t)(std::max(g.n, g.nb) + 255) / 256 + 1));
-        if (!std::is_same<R, double>::value) { TRY(alloc(a.p4, n)); TRY(alloc(a.v4, n)); }
+        if (!std::is_same<R, double>::value) {
+            TRY(alloc(a.p4, n)); TRY(alloc(a.v4, n));
+            TRY(alloc(a.wact, n - g.wb));
+        }
         return DMPH_OK;
     }
 
@@ -535,7 +539,11 @@ class Engine final : public dmph_handle {
             if (body[i] >= 0) mrank[i] = moff[body[i] + 1]++;
             else g.nloose += is_solid(gr.type[i]);
         }
+        int most = 0;
+        for (int b = 0; b < nb; b++) most = std::max(most, moff[b + 1]);
         for (int b = 0; b < nb; b++) moff[b + 1] += moff[b];
+        g.body_tpb = 64;   // one member per thread up to 1024 members
+        while (g.body_tpb < most && g.body_tpb < 1024) g.body_tpb *= 2;
         TRY(alloc(a.mrank, n)); TRY(alloc(a.b_moff, nb + 1)); TRY(alloc(a.b_mslot, std::max(moff[nb], 1)));
         HCHK(hipMemcpyAsync(a.mrank, mrank.data(), n * sizeof(int), hipMemcpyHostToDevice, stream));
         HCHK(hipMemcpyAsync(a.b_moff, moff.data(), (nb + 1) * sizeof(int), hipMemcpyHostToDevice, stream));
@@ -899,6 +907,10 @@ class Engine final : public dmph_handle {
         if (field == DMPH_FIELD_POSITION) {
             const int one = 1;
             HCHK(hipMemcpyAsync(&a.sc->force, &one, sizeof one, hipMemcpyHostToDevice, stream));
+        }
+        {   // the next step's sweep 1 covers every particle (walls may have been moved or set in motion)
+            const int two = 2;
+            HCHK(hipMemcpyAsync(&a.sc->full_sweep, &two, sizeof two, hipMemcpyHostToDevice, stream));
         }
         HCHK(hipStreamSynchronize(stream));   // host vectors go out of scope
         return DMPH_OK;
