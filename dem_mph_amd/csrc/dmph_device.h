@@ -43,6 +43,7 @@ struct Geo {
     int lanes[2];       // FP32 sweeps 1 and 2: lanes per particle (2, 4, 8 or 16)
     int block[2];       // FP32 sweeps 1 and 2: threads per workgroup (64, 128 or 256)
     int fuse_accel;     // FP32: the fluid sweep 2 also applies calculateAcceleration (0 while re-running sweeps)
+    int fused;          // FP32 without surface tension: passes A and B in place of sweeps 1 and 2
     int nloose;         // solid particles outside rigid bodies (moved by k_move_solids)
     int ccap;           // DEM contact-list capacity per particle
     int body_f32;       // FP32 only: rigid-body sums and integration in FP32 (the plain solver)
@@ -105,6 +106,7 @@ struct Dev {
     double *blk_max;               // per-block maxima of the squared displacement
     float4 *p4, *v4;               // FP32 sweeps: packed (position, type) and (velocity, PressureP)
     int *wact;                     // FP32, per wall slot: its row at the last rebinning holds a fluid or solid particle
+    float4 *pp4;                   // FP32 passes A / B: (position, PressureP) written by pass A
     const double *x64;             // FP32, first binning only: the input positions [n][3] in original order
     Scalars *sc;
 };

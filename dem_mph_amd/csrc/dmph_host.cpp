@@ -283,6 +283,7 @@ class Engine final : public dmph_handle {
         g.block[0] = g.block[1] = (o.block == 64 || o.block == 128) ? o.block : 256;
         g.body_f32 = o.rigid_precision == DMPH_PREC_FP32;
         g.fuse_accel = 1;
+        g.fused = getenv("DMPH_SWEEPS") && !strcmp(getenv("DMPH_SWEEPS"), "separate") ? 0 : 1;
         g.sort_reference = o.sort_mode == DMPH_SORT_REFERENCE;
         g.rebuild_every = o.rebuild == DMPH_REBUILD_EVERY_STEP;
         weights(g, p);
@@ -409,6 +410,7 @@ class Engine final : public dmph_handle {
         if (!std::is_same<R, double>::value) {
             TRY(alloc(a.p4, n)); TRY(alloc(a.v4, n));
             TRY(alloc(a.wact, n - g.wb));
+            TRY(alloc(a.pp4, n));
         }
         return DMPH_OK;
     }
@@ -708,6 +710,7 @@ class Engine final : public dmph_handle {
         HCHK(hipMemcpyAsync(a.fy, save + n, n * sizeof(R), hipMemcpyDeviceToDevice, stream));
         HCHK(hipMemcpyAsync(a.fz, save + 2 * n, n * sizeof(R), hipMemcpyDeviceToDevice, stream));
         g.fuse_accel = 1;
+        g.fused = getenv("DMPH_SWEEPS") && !strcmp(getenv("DMPH_SWEEPS"), "separate") ? 0 : 1;
         HCHK(hipStreamSynchronize(stream));
         HCHK(hipGetLastError());
         // launch arguments changed: capture the step again at the next call
