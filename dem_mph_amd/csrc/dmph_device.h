@@ -114,7 +114,8 @@ struct Dev {
 // Launchers (defined in dmph_kernels.inc, instantiated for float and double).
 template <class R> void launch_predict(const Geo &g, const Dev<R> &a, hipStream_t s);
 template <class R> void launch_rebin(const Geo &g, const Dev<R> &a, hipStream_t s);
-template <class R> void launch_sweep1(const Geo &g, const Dev<R> &a, hipStream_t s);
+// s_solid: the stream of the solids' pass A (FP32 fused passes; null = s)
+template <class R> void launch_sweep1(const Geo &g, const Dev<R> &a, hipStream_t s, hipStream_t s_solid = nullptr);
 template <class R> void launch_sweep2(const Geo &g, const Dev<R> &a, hipStream_t s);
 template <class R> void launch_accel(const Geo &g, const Dev<R> &a, hipStream_t s);
 template <class R> void launch_dem(const Geo &g, const Dev<R> &a, hipStream_t s);
@@ -124,7 +125,7 @@ template <class R> void launch_init_density(const Geo &g, const Dev<R> &a, hipSt
 template <class R> void launch_export(const Geo &g, const Dev<R> &a, float *pos, float *vel, hipStream_t s);
 
 // FP32 sweeps (dmph_sweep32.inc, in the FP32 object only)
-void launch_sweep1_f32(const Geo &g, const Dev<float> &a, hipStream_t s, int init);
+void launch_sweep1_f32(const Geo &g, const Dev<float> &a, hipStream_t s, int init, hipStream_t s_solid = nullptr);
 void launch_sweep2_f32(const Geo &g, const Dev<float> &a, hipStream_t s);
 
 }  // namespace dmph

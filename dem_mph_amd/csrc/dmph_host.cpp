@@ -217,6 +217,9 @@ class Engine final : public dmph_handle {
     dmph_params p{};
     dmph_options o{};
     hipStream_t stream = nullptr;
+    // the second branch of a captured FP32 step (solids' pass A, DEM, body update)
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_solid = nullptr, ev_join = nullptr;
     std::vector<void *> allocs;
     int device = 0;
     double spacing = 0;
@@ -230,6 +233,9 @@ class Engine final : public dmph_handle {
         for (auto &p_ : pending) { ev_pool.push_back(p_.second.first); ev_pool.push_back(p_.second.second); }
         for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
         for (void *q : allocs) (void)hipFree(q);
+        for (hipEvent_t e : {ev_fork, ev_solid, ev_join})
+            if (e) (void)hipEventDestroy(e);
+        if (side) (void)hipStreamDestroy(side);
         if (stream) (void)hipStreamDestroy(stream);
     }
 
@@ -264,6 +270,10 @@ class Engine final : public dmph_handle {
         device = o.device;
         HCHK(hipSetDevice(device));
         HCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        HCHK(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+        HCHK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+        HCHK(hipEventCreateWithFlags(&ev_solid, hipEventDisableTiming));
+        HCHK(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
         g.n = n;
         g.d = spacing = gr->spacing;
         for (int k = 0; k < 3; k++) dmax[k] = gr->domain_max[k];
@@ -620,9 +630,39 @@ class Engine final : public dmph_handle {
     // One pass of the loop body: every launch of a step, in order.  No host
     // synchronisation and no memset: the rebin decision is a device flag that
     // the rebin kernels read, so the same sequence serves every step.
-    void enqueue_step() {
+    //
+    // Captured FP32 steps with solid slots and the fused passes run in two
+    // graph branches after the rebin kernels: the fluid and wall slots' pass A
+    // then pass B on the main stream; the solid slots' pass A, the DEM
+    // sub-steps and the body update beside them.  Pass B reads the solids'
+    // (position, PressureP) from their pass A, so it waits for that launch;
+    // the rest of the solid branch reads nothing the fluid and wall passes write
+    // (the DEM pass takes a fluid or wall contact's Mass from Density x Volume,
+    // its force loop reads solid and wall slots only) and writes only solid and
+    // wall slots' forces and kinematics, which those passes do not read (they
+    // read the packed copies k_predict wrote).  The step ends when both
+    // branches have.
+    bool capturing = false;
+    int enqueue_step() {
         timed(DMPH_TIMING_PREDICT, [&] { launch_predict<R>(g, a, stream); });
         timed(DMPH_TIMING_REBIN, [&] { launch_rebin<R>(g, a, stream); });
+        const bool branch = capturing && !std::is_same<R, double>::value && g.fused && !g.st_active &&
+                            g.se > g.sb && !(getenv("DMPH_BRANCHES") && !strcmp(getenv("DMPH_BRANCHES"), "serial"));
+        if (branch) {
+            HCHK(hipEventRecord(ev_fork, stream));
+            HCHK(hipStreamWaitEvent(side, ev_fork, 0));
+            launch_sweep1<R>(g, a, stream, side);
+            HCHK(hipEventRecord(ev_solid, side));
+            for (int s = 0; s < g.substeps; s++) {
+                launch_dem<R>(g, a, side);
+                launch_move<R>(g, a, side);
+            }
+            HCHK(hipEventRecord(ev_join, side));
+            HCHK(hipStreamWaitEvent(stream, ev_solid, 0));
+            launch_sweep2<R>(g, a, stream);
+            HCHK(hipStreamWaitEvent(stream, ev_join, 0));
+            return DMPH_OK;
+        }
         timed(DMPH_TIMING_SWEEP1, [&] { launch_sweep1<R>(g, a, stream); });
         timed(DMPH_TIMING_SWEEP2, [&] { launch_sweep2<R>(g, a, stream); });
         // FP32 applies the fluid acceleration inside sweep 2: no group of its own
@@ -632,6 +672,7 @@ class Engine final : public dmph_handle {
             timed(DMPH_TIMING_DEM, [&] { launch_dem<R>(g, a, stream); });
             timed(DMPH_TIMING_MOVE, [&] { launch_move<R>(g, a, stream); });
         }
+        return DMPH_OK;
     }
 
     // The step as a HIP graph, captured on first use and replayed once per step.
@@ -640,9 +681,15 @@ class Engine final : public dmph_handle {
 
     int capture_step() {
         HCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-        enqueue_step();
+        capturing = true;
+        const int rc = enqueue_step();
+        capturing = false;
         hipGraph_t gr = nullptr;
         const hipError_t e = hipStreamEndCapture(stream, &gr);
+        if (rc != DMPH_OK) {
+            if (gr) (void)hipGraphDestroy(gr);
+            return rc;
+        }
         if (e != hipSuccess || !gr) {
             err = std::string("hipStreamEndCapture: ") + hipGetErrorString(e);
             return DMPH_E_DEVICE;
@@ -730,7 +777,7 @@ class Engine final : public dmph_handle {
             if (use_graph) {
                 HCHK(hipGraphLaunch(graph_exec, stream));
             } else {
-                enqueue_step();
+                TRY(enqueue_step());
                 if (timing && pending.size() > 4096) TRY(resolve_timing());
             }
         }
