@@ -111,10 +111,11 @@ VARIANTS = {
 }
 
 
-def errors_vs_oracle(D, sc, params, variant, o, steps):
-    """Per-field errors of a fresh GPU run of `steps` steps against the FP64
-    oracle at the same step, plus mass and momentum balance of both."""
-    s = D.Solver(D.Params.from_dict(params), sc, rigid=True, **VARIANTS[variant])
+def errors_vs_oracle(D, sc, params, variant, o, steps, cell_scale=1.0):
+    """Per-field errors of a fresh GPU run of `steps` steps (the variant's cell
+    size) against the FP64 oracle at the same step, plus mass and momentum
+    balance of both."""
+    s = D.Solver(D.Params.from_dict(params), sc, rigid=True, cell_scale=cell_scale, **VARIANTS[variant])
     s.step(steps)
     s.sync()
     typ = o.field("type")
@@ -174,8 +175,9 @@ def time_variant(D, sc, params, variant, steps, warmup):
     """Timed region: `steps` replays of the captured step (hipGraph), bracketed
     by device synchronisation.  Then the same number of steps launched one by
     one with HIP events around each kernel group, on the library's stream, for
-    the per-kernel durations."""
-    s = D.Solver(D.Params.from_dict(params), sc, rigid=True, graph=True, **VARIANTS[variant])
+    the per-kernel durations.  FP32 variants tune the cell size at creation
+    (cell_scale < 0: each candidate width on a handle of its own)."""
+    s = D.Solver(D.Params.from_dict(params), sc, rigid=True, graph=True, cell_scale=-1.0, **VARIANTS[variant])
     info = s.info()
     s.step(warmup)
     s.sync()
@@ -198,13 +200,14 @@ def time_variant(D, sc, params, variant, steps, warmup):
     blocks = list(s.info().block)
     s.close()
     return {"seconds": el, "timing": tm, "info": info, "rebuilds": int(reb - reb0), "seconds_timed_launches": el_ev,
-            "lanes": list(lanes), "blocks": blocks}
+            "lanes": list(lanes), "blocks": blocks, "cell_scale": float(info.cell_scale)}
 
 
-def full_config(D, sc, params, variant, steps):
+def full_config(D, sc, params, variant, steps, cell_scale=1.0):
     """The whole C2 configuration as stated: `steps` steps from t = 0 (every
-    rebuild of the run included), after autotuning on the initial state."""
-    s = D.Solver(D.Params.from_dict(params), sc, rigid=True, graph=True, **VARIANTS[variant])
+    rebuild of the run included), after autotuning on the initial state, with
+    the cell size the variant's tuning chose."""
+    s = D.Solver(D.Params.from_dict(params), sc, rigid=True, graph=True, cell_scale=cell_scale, **VARIANTS[variant])
     lanes = s.autotune(3)
     s.step(1)   # the graph capture, outside the clock; the run below starts at step 1
     s.sync()
@@ -219,7 +222,7 @@ def full_config(D, sc, params, variant, steps):
     dt = params["Dt"]
     return {"steps": steps, "timed_steps": steps - 1, "ms_per_step": el / (steps - 1) * 1e3,
             "particle_steps_per_s": sc.n * (steps - 1) / el, "sim_s_per_wall_s": dt * (steps - 1) / el,
-            "rebuilds": int(reb), "simulated_s": t_end, "autotuned_lanes": list(lanes)}
+            "rebuilds": int(reb), "simulated_s": t_end, "autotuned_lanes": list(lanes), "cell_scale": cell_scale}
 
 
 def ensemble_bench(D, nscenes: int, steps: int, warmup: int, seed: int, sample: int):
@@ -237,10 +240,12 @@ def ensemble_bench(D, nscenes: int, steps: int, warmup: int, seed: int, sample: 
                        "1-32 rigid cuboids of side 3-8, dt = 5e-5 s" % nscenes,
            "scenes": nscenes, "particles": n, "steps": steps, "generation_s": t_gen}
     res = {}
+    cs = -1.0   # the mixed handle tunes the cell size at creation; the plain one reuses its choice
     for name, var in (("batched_mixed", "mixed"), ("batched_plain", "plain")):
         t0 = time.perf_counter()
-        s = D.Solver(D.Params.from_dict(params), scenes, rigid=True, **VARIANTS[var])
+        s = D.Solver(D.Params.from_dict(params), scenes, rigid=True, cell_scale=cs, **VARIANTS[var])
         t_create = time.perf_counter() - t0
+        cs = float(s.info().cell_scale)
         s.step(warmup)
         s.sync()
         lanes = s.autotune(3)
@@ -252,7 +257,8 @@ def ensemble_bench(D, nscenes: int, steps: int, warmup: int, seed: int, sample: 
         el = time.perf_counter() - t0
         s.close()
         res[name] = {"particle_steps_per_s": n * steps / el, "ms_per_step": el / steps * 1e3,
-                     "sim_s_per_wall_s": dt * steps / el, "create_s": t_create, "autotuned_lanes": list(lanes)}
+                     "sim_s_per_wall_s": dt * steps / el, "create_s": t_create, "autotuned_lanes": list(lanes),
+                     "cell_scale": cs}
         print("bench: ensemble %s %.2f ms/step" % (name, el / steps * 1e3), file=sys.stderr, flush=True)
     # plain FP32 solver, one scene per handle, scenes one after another (each autotuned)
     idx = np.linspace(0, nscenes - 1, min(sample, nscenes)).round().astype(int)
@@ -489,7 +495,8 @@ def main(argv=None) -> int:
                          "sim_s_per_wall_s": dt * (a.fp64_steps if k == "fp64" else a.steps) / v["seconds"],
                          "steps": a.fp64_steps if k == "fp64" else a.steps,
                          "options": {kk: vv for kk, vv in VARIANTS[k].items()},
-                         "autotuned_lanes": v["lanes"], "autotuned_blocks": v["blocks"]}
+                         "autotuned_lanes": v["lanes"], "autotuned_blocks": v["blocks"],
+                         "autotuned_cell_scale": v["cell_scale"]}
                      for k, v in variants.items()},
         "roofline": {"bound": "hbm", "kernel": "sweeps: pass A (%s, %s) + pass B (%s)" % tuple(knames),
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -505,13 +512,16 @@ def main(argv=None) -> int:
     }
     if a.config_steps > 0 and prec == D.PREC_FP32:
         print("bench: the whole C2 configuration, %d steps from t = 0" % a.config_steps, file=sys.stderr, flush=True)
-        out["full_config"] = {v: full_config(D, sc, params, v, a.config_steps) for v in ("mixed", "plain")}
+        out["full_config"] = {v: full_config(D, sc, params, v, a.config_steps,
+                                             variants[v]["cell_scale"] if v in variants else 1.0)
+                              for v in ("mixed", "plain")}
     if not a.no_cpu_baseline:
         print("bench: timing the CPU baseline (bounded sample)", file=sys.stderr, flush=True)
         rec, o, k = cpu_baseline(sc, params, a.cpu_budget)
         out["cpu_baseline"] = rec
         print("bench: errors against the FP64 oracle after %d steps" % k, file=sys.stderr, flush=True)
-        out["errors_vs_fp64_oracle"] = {v: errors_vs_oracle(D, sc, params, v, o, k) for v in variants}
+        out["errors_vs_fp64_oracle"] = {v: errors_vs_oracle(D, sc, params, v, o, k, variants[v]["cell_scale"])
+                                        for v in variants}
         o.close()
     else:
         out["cpu_baseline"] = None

@@ -235,7 +235,9 @@ class Solver:
     """One DEM-MPH scene on one GPU, driven through the C ABI -- or, given a
     list of scenes that share spacing and domain, a batch advanced together
     (dmph_create_batch); fields then cover all scenes, scene after scene
-    (`scene_offsets`)."""
+    (`scene_offsets`).  FP32 options: `lanes` and `block` (see `autotune`),
+    `cell_scale` (cell width over the reference's, 1 to 2; negative: tuned at
+    creation over 1, 1.25 and 1.5, reported by `info().cell_scale`)."""
 
     def __init__(self, params: Params, scene, precision: int = PREC_FP64, sort_mode: int = SORT_CANONICAL,
                  rebuild: int = REBUILD_SKIN, rigid: bool = False, dim: Optional[int] = None, device: int = 0,

@@ -7,6 +7,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <algorithm>
+#include <chrono>
 #include <memory>
 #include <string>
 #include <type_traits>
@@ -678,6 +679,10 @@ class Engine final : public dmph_handle {
     // The step as a HIP graph, captured on first use and replayed once per step.
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
+    void drop_graph() {
+        if (graph_exec) { (void)hipGraphExecDestroy(graph_exec); graph_exec = nullptr; }
+        if (graph) { (void)hipGraphDestroy(graph); graph = nullptr; }
+    }
 
     int capture_step() {
         HCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
@@ -1103,32 +1108,99 @@ void dmph_default_options(dmph_options *opt) {
     opt->graph = 1;
 }
 
-int dmph_create(const dmph_params *p, const dmph_options *opt, const dmph_grid *g, dmph_handle **out) {
-    if (!p || !g || !out) return DMPH_E_ARG;
-    *out = nullptr;
-    dmph_options o;
-    if (opt) o = *opt;
-    else dmph_default_options(&o);
-    std::unique_ptr<dmph_handle> h;
+// One handle of the options' precision over grid g (scene_of / nscene: a batch).
+static int make_handle(const dmph_params *p, const dmph_options &o, const dmph_grid *g, const int *scene_of,
+                       int nscene, std::unique_ptr<dmph_handle> &h) {
     int rc;
     if (o.precision == DMPH_PREC_FP32) {
         auto e = std::make_unique<dmph::Engine<float>>();
-        rc = e->create(p, &o, g);
+        rc = e->create(p, &o, g, scene_of, nscene);
         h = std::move(e);
     } else if (o.precision == DMPH_PREC_FP64) {
         auto e = std::make_unique<dmph::Engine<double>>();
-        rc = e->create(p, &o, g);
+        rc = e->create(p, &o, g, scene_of, nscene);
         h = std::move(e);
     } else {
         g_create_error = "unknown precision";
         return DMPH_E_ARG;
     }
-    if (rc != DMPH_OK) {
-        g_create_error = h->err;
-        return rc;
+    if (rc != DMPH_OK) g_create_error = h->err;
+    return rc;
+}
+
+// Cell-size tuner (FP32, dmph_options.cell_scale < 0): each candidate width is
+// tried on a handle of its own -- created, lanes and workgroups autotuned, a
+// few steps of warm-up -- timed over replayed steps between rebuilds and over
+// steps that rebin every step, and scored as the step time plus the extra
+// time of a rebuild step spread over kTuneInterval steps (the dam breaks
+// rebuild every ~20 steps).  The candidates do not change which pairs a row
+// holds, only the cell grid, the slot order and the list-build cost.
+static const double kTuneCells[] = {1.0, 1.25, 1.5};
+constexpr int kTuneSteps = 20, kTuneRebuilds = 4, kTuneInterval = 20;
+
+static int tune_cells(const dmph_params *p, dmph_options o, const dmph_grid *g, const int *scene_of, int nscene,
+                      double *best) {
+    using clk = std::chrono::steady_clock;
+    auto secs = [](clk::time_point a, clk::time_point b) { return std::chrono::duration<double>(b - a).count(); };
+    double best_score = 1e300;
+    *best = 1.0;
+    for (double c : kTuneCells) {
+        o.cell_scale = c;
+        auto e = std::make_unique<dmph::Engine<float>>();
+        int rc = e->create(p, &o, g, scene_of, nscene);
+        if (rc == DMPH_OK) rc = e->step(5);
+        if (rc == DMPH_OK) rc = e->sync();
+        if (rc == DMPH_OK) rc = e->autotune(2, nullptr);
+        if (rc == DMPH_OK) rc = e->step(2);
+        if (rc == DMPH_OK) rc = e->sync();
+        const auto t0 = clk::now();
+        if (rc == DMPH_OK) rc = e->step(kTuneSteps);
+        if (rc == DMPH_OK) rc = e->sync();
+        const auto t1 = clk::now();
+        e->g.rebuild_every = 1;   // the same launches, every step a rebinning step
+        e->drop_graph();
+        if (rc == DMPH_OK) rc = e->step(1);
+        if (rc == DMPH_OK) rc = e->sync();
+        const auto t2 = clk::now();
+        if (rc == DMPH_OK) rc = e->step(kTuneRebuilds);
+        if (rc == DMPH_OK) rc = e->sync();
+        const auto t3 = clk::now();
+        if (rc != DMPH_OK) {
+            g_create_error = "cell-size tuning at cell_scale " + std::to_string(c) + ": " + e->err;
+            return rc;
+        }
+        const double step = secs(t0, t1) / kTuneSteps, rebuild = secs(t2, t3) / kTuneRebuilds;
+        const double score = step + std::max(0.0, rebuild - step) / kTuneInterval;
+        if (score < best_score) { best_score = score; *best = c; }
     }
+    return DMPH_OK;
+}
+
+// dmph_create and dmph_create_batch: the handle, with the cell size tuned first when asked
+static int create_handle(const dmph_params *p, const dmph_options *opt, const dmph_grid *g, const int *scene_of,
+                         int nscene, dmph_handle **out) {
+    dmph_options o;
+    if (opt) o = *opt;
+    else dmph_default_options(&o);
+    if (o.cell_scale < 0.0) {
+        double c = 1.0;   // FP64 keeps the reference's cell grid
+        if (o.precision == DMPH_PREC_FP32) {
+            const int rc = tune_cells(p, o, g, scene_of, nscene, &c);
+            if (rc != DMPH_OK) return rc;
+        }
+        o.cell_scale = c;
+    }
+    std::unique_ptr<dmph_handle> h;
+    const int rc = make_handle(p, o, g, scene_of, nscene, h);
+    if (rc != DMPH_OK) return rc;
     *out = h.release();
     return DMPH_OK;
+}
+
+int dmph_create(const dmph_params *p, const dmph_options *opt, const dmph_grid *g, dmph_handle **out) {
+    if (!p || !g || !out) return DMPH_E_ARG;
+    *out = nullptr;
+    return create_handle(p, opt, g, nullptr, 1, out);
 }
 
 int dmph_create_batch(const dmph_params *p, const dmph_options *opt, const dmph_grid *grids, int nscenes,
@@ -1185,29 +1257,7 @@ int dmph_create_batch(const dmph_params *p, const dmph_options *opt, const dmph_
     all.vel = vel.data();
     all.body = bodies ? body.data() : nullptr;
     all.density = dens ? rho.data() : nullptr;
-    dmph_options o;
-    if (opt) o = *opt;
-    else dmph_default_options(&o);
-    std::unique_ptr<dmph_handle> h;
-    int rc;
-    if (o.precision == DMPH_PREC_FP32) {
-        auto e = std::make_unique<dmph::Engine<float>>();
-        rc = e->create(p, &o, &all, scene.data(), nscenes);
-        h = std::move(e);
-    } else if (o.precision == DMPH_PREC_FP64) {
-        auto e = std::make_unique<dmph::Engine<double>>();
-        rc = e->create(p, &o, &all, scene.data(), nscenes);
-        h = std::move(e);
-    } else {
-        g_create_error = "unknown precision";
-        return DMPH_E_ARG;
-    }
-    if (rc != DMPH_OK) {
-        g_create_error = h->err;
-        return rc;
-    }
-    *out = h.release();
-    return DMPH_OK;
+    return create_handle(p, opt, &all, scene.data(), nscenes, out);
 }
 
 int dmph_step(dmph_handle *h, int nsteps) { return h && nsteps >= 0 ? h->step(nsteps) : DMPH_E_ARG; }

@@ -81,7 +81,10 @@ typedef struct dmph_options {
     double cell_scale;  /* FP32 only: cell width = cell_scale x (MaxRadius + 0.1 d), cell_scale in [1, 2]
                            (0 or 1: the reference's cell grid, source/DEM-MPH.cpp:1211-1239); the 3^D stencil
                            still covers the cutoff, rows hold the same pairs, slot order and list-build
-                           cost change */
+                           cost change.  Negative: tuned at creation -- 1, 1.25 and 1.5 are each tried on a
+                           handle of their own (autotuned, timed between rebuilds and on rebuild steps, scored
+                           as step time + rebuild cost over 20 steps) and the handle is created with the
+                           best; dmph_get_info reports it (FP64: the reference's grid) */
 } dmph_options;
 
 void dmph_default_options(dmph_options *opt);
@@ -248,8 +251,8 @@ int dmph_get_timing(dmph_handle *h, int group, double *total_ms, int64_t *launch
  * (lanes, workgroup) for each sweep; the step graph is captured again.  The
  * choice changes FP32 summation order only.  FP64 handles keep one thread per
  * particle.  Writes the chosen lanes to lanes[2] if not NULL; dmph_get_info
- * reports lanes and workgroup sizes.  The cell size (dmph_options.cell_scale)
- * needs a new handle and is tuned by dem_mph_amd.autotune. */
+ * reports lanes and workgroup sizes.  The cell size needs a new handle: it is
+ * tuned at creation when dmph_options.cell_scale is negative. */
 int dmph_autotune(dmph_handle *h, int reps, int *lanes);
 
 const char *dmph_last_error(const dmph_handle *h);

@@ -346,6 +346,30 @@ def test_fp32_workgroups_and_cell_sizes(name, block, cell_scale):
     s.close(); o.close()
 
 
+def test_cell_size_tuned_at_creation():
+    """cell_scale < 0 tunes the cell width at creation (FP32): the handle comes
+    back on one of the candidate grids (1, 1.25 or 1.5 x the reference's cell),
+    at its initial state -- the candidates' trial steps leave no trace: the
+    same step count and state as a handle created directly on that grid -- and
+    FP64 keeps the reference's grid."""
+    sc, prm, L = _case("mix3d")
+    p = D.Params.from_dict(prm)
+    s = D.Solver(p, sc, precision=D.PREC_FP32, cell_scale=-1.0)
+    c = float(s.info().cell_scale)
+    assert min(abs(c - x) for x in (1.0, 1.25, 1.5)) < 1e-12
+    ref = D.Solver(p, sc, precision=D.PREC_FP32, cell_scale=c)
+    assert s.time() == ref.time() == sc.time
+    for k in (1, 3):
+        s.step(k); ref.step(k)
+        s.sync(); ref.sync()
+        for f in ("pos", "vel", "force", "key"):
+            np.testing.assert_array_equal(s.field(f), ref.field(f), err_msg=f)
+    s.close(); ref.close()
+    s64 = D.Solver(p, sc, precision=D.PREC_FP64, cell_scale=-1.0)
+    assert float(s64.info().cell_scale) == 1.0
+    s64.close()
+
+
 @pytest.mark.parametrize("prec", [D.PREC_FP64, D.PREC_FP32])
 def test_rigid_body_pose_over_1000_steps(prec):
     """The debris of mix3d as one rigid body in the fluid (d = 0.02 m), 1000
