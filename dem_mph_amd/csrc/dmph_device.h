@@ -106,6 +106,8 @@ struct Dev {
     double *blk_max;               // per-block maxima of the squared displacement
     float4 *p4, *v4;               // FP32 sweeps: packed (position, type) and (velocity, PressureP)
     int *wact;                     // FP32, per wall slot: its row at the last rebinning holds a fluid or solid particle
+    int *wnfs;                     // FP32, per wall slot: near entries of fluid and solid particles (the row's first ones)
+    double *wsq;                   // FP32, per wall slot: its wall neighbours' density sum, kept while walls stand still
     float4 *pp4;                   // FP32 passes A / B: (position, PressureP) written by pass A
     const double *x64;             // FP32, first binning only: the input positions [n][3] in original order
     Scalars *sc;

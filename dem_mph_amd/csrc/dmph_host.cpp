@@ -421,6 +421,8 @@ class Engine final : public dmph_handle {
         if (!std::is_same<R, double>::value) {
             TRY(alloc(a.p4, n)); TRY(alloc(a.v4, n));
             TRY(alloc(a.wact, n - g.wb));
+            TRY(alloc(a.wnfs, n - g.wb));
+            TRY(alloc(a.wsq, n - g.wb));
             TRY(alloc(a.pp4, n));
         }
         return DMPH_OK;

@@ -346,6 +346,36 @@ def test_fp32_workgroups_and_cell_sizes(name, block, cell_scale):
     s.close(); o.close()
 
 
+def test_static_wall_density_between_full_sweeps():
+    """Static walls keep their wall neighbours' density sum from the last full
+    sweep and walk only their rows' fluid and solid entries on the other steps.
+    mix3d's walls stand still; the first two steps sweep everything, steps 3-6
+    do not (no rebinning: the skin holds).  After every step the walls' number
+    density stays within 1e-5 per particle of the FP64 oracle, as the fluid's,
+    and the walls' PressureP within 1e-4 norm-relative."""
+    g = G.load("mix3d")
+    sc, prm = G.scene(g), G.params(g)
+    s = D.Solver(D.Params.from_dict(prm), sc, precision=D.PREC_FP32, sort_mode=D.SORT_CANONICAL, rigid=False)
+    o = P.Oracle(P.params_from_dict(prm), sc, sort_mode=P.SORT_CANONICAL)
+    n0p = o.scalars()["n0p"]
+    typ = o.field("type")
+    wall = typ >= 4
+    assert wall.any()
+    reb2 = None
+    for k in range(6):
+        s.step(1); o.step(1)
+        if k == 1:
+            reb2 = s.info().rebuilds
+        nd, ndr = s.field("vol_strain_p") + n0p, o.field("vol_strain_p") + n0p
+        rel = np.abs(nd - ndr) / np.abs(ndr)
+        pw, pwr = s.field("pressure_p")[wall], o.field("pressure_p")[wall]
+        prel = np.linalg.norm(pw - pwr) / max(np.linalg.norm(pwr), 1e-300)
+        assert rel[wall].max() <= 1e-5 and rel.max() <= 1e-5, (k, rel[wall].max(), rel.max())
+        assert prel <= 1e-4, (k, prel)
+    assert s.info().rebuilds == reb2   # steps 3-6 rebinned nothing: the partial wall walk ran
+    s.close(); o.close()
+
+
 def test_cell_size_tuned_at_creation():
     """cell_scale < 0 tunes the cell width at creation (FP32): the handle comes
     back on one of the candidate grids (1, 1.25 or 1.5 x the reference's cell),
