@@ -108,6 +108,7 @@ struct Dev {
     int *wact;                     // FP32, per wall slot: its row at the last rebinning holds a fluid or solid particle
     int *wnfs;                     // FP32, per wall slot: near entries of fluid and solid particles (the row's first ones)
     double *wsq;                   // FP32, per wall slot: its wall neighbours' density sum, kept while walls stand still
+    int *wdem;                     // FP32, per wall slot: its last DEM contact is a wall (coefficients kept while static)
     float4 *pp4;                   // FP32 passes A / B: (position, PressureP) written by pass A
     const double *x64;             // FP32, first binning only: the input positions [n][3] in original order
     Scalars *sc;

@@ -423,6 +423,7 @@ class Engine final : public dmph_handle {
             TRY(alloc(a.wact, n - g.wb));
             TRY(alloc(a.wnfs, n - g.wb));
             TRY(alloc(a.wsq, n - g.wb));
+            TRY(alloc(a.wdem, n - g.wb));
             TRY(alloc(a.pp4, n));
         }
         return DMPH_OK;

@@ -376,6 +376,28 @@ def test_static_wall_density_between_full_sweeps():
     s.close(); o.close()
 
 
+def test_static_wall_dem_coefficients_kept():
+    """A static wall whose last DEM contact is a wall keeps its coefficients
+    between full sweeps instead of recomputing them.  Two identical FP32 runs
+    of mix3d; after 6 steps the second writes its positions back (a rebinning
+    and full sweeps follow), the first keeps going: after one more step every
+    wall's four coefficients are equal bit for bit."""
+    g = G.load("mix3d")
+    sc, prm = G.scene(g), G.params(g)
+    a = D.Solver(D.Params.from_dict(prm), sc, precision=D.PREC_FP32)
+    b = D.Solver(D.Params.from_dict(prm), sc, precision=D.PREC_FP32)
+    a.step(6); b.step(6)
+    a.sync(); b.sync()
+    b.set_field("pos", b.field("pos"))
+    a.step(1); b.step(1)
+    a.sync(); b.sync()
+    assert b.info().rebuilds == a.info().rebuilds + 1
+    wall = a.field("type") >= 4
+    for f in ("kn", "etan", "kt", "etat"):
+        np.testing.assert_array_equal(a.field(f)[wall], b.field(f)[wall], err_msg=f)
+    a.close(); b.close()
+
+
 def test_cell_size_tuned_at_creation():
     """cell_scale < 0 tunes the cell width at creation (FP32): the handle comes
     back on one of the candidate grids (1, 1.25 or 1.5 x the reference's cell),
