@@ -70,7 +70,7 @@ struct Scalars {
     int list_overflow;              // longest row that did not fit the neighbour list (0 = none)
     double wall_center[2][3];
     unsigned int ticket[4];         // blocks finished, per kernel that ends in a last-block phase
-    int full_sweep;                 // FP32: > 0 -> sweep 1 covers every wall particle (k_decide counts it down)
+    int full_sweep;                 // FP32: > 0 -> sweep 1 covers every wall particle (the key pass counts it down)
     int walls_moving;               // this step moved the walls (0.2 <= Time <= 0.4, written by k_predict)
 };
 
@@ -103,7 +103,8 @@ struct Dev {
     double *b_mass, *b_x, *b_v, *b_l, *b_q, *b_iinv, *b_w, *b_r;
     int *mrank;                    // [n] rank of a member within its body (fixed at set-up, moves with the particle)
     int *b_moff, *b_mslot;         // per body: offsets into b_mslot [nb + 1]; member slots [members], by rank
-    double *blk_max;               // per-block maxima of the squared displacement
+    unsigned long long *skin_part; // [8 x 16] max squared displacement this step, one slot per XCD (bits of a
+                                   // non-negative double, 128 B apart), zeroed by the scan after the key pass reads it
     float4 *p4, *v4;               // FP32 sweeps: packed (position, type) and (velocity, PressureP)
     int *wact;                     // FP32, per wall slot: its row at the last rebinning holds a fluid or solid particle
     int *wnfs;                     // FP32, per wall slot: near entries of fluid and solid particles (the row's first ones)
@@ -116,7 +117,8 @@ struct Dev {
 
 // Launchers (defined in dmph_kernels.inc, instantiated for float and double).
 template <class R> void launch_predict(const Geo &g, const Dev<R> &a, hipStream_t s);
-template <class R> void launch_rebin(const Geo &g, const Dev<R> &a, hipStream_t s);
+// decide = 1: the key pass takes the rebin decision of a step; 0: the flag is already set (first binning)
+template <class R> void launch_rebin(const Geo &g, const Dev<R> &a, hipStream_t s, int decide);
 // s_solid: the stream of the solids' pass A (FP32 fused passes; null = s)
 template <class R> void launch_sweep1(const Geo &g, const Dev<R> &a, hipStream_t s, hipStream_t s_solid = nullptr);
 template <class R> void launch_sweep2(const Geo &g, const Dev<R> &a, hipStream_t s);

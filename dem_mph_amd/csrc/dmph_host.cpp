@@ -388,7 +388,7 @@ class Engine final : public dmph_handle {
             if (e != hipSuccess) { (void)hipFree(x64); HCHK(e); }
             a.x64 = x64;
         }
-        launch_rebin<R>(g, a, stream);
+        launch_rebin<R>(g, a, stream, 0);
         launch_init_density<R>(g, a, stream);
         a.x64 = nullptr;
         const hipError_t e1 = hipGetLastError(), e2 = hipStreamSynchronize(stream);
@@ -417,7 +417,7 @@ class Engine final : public dmph_handle {
         TRY(alloc(a.t_int, 5 * n)); TRY(alloc(a.t_real, 8 * n));
         TRY(alloc(a.t_dbl, g.rigid ? 3 * n : 1)); TRY(alloc(a.r0, g.rigid ? 3 * n : 1));
         TRY(alloc(a.sc, 1));
-        TRY(alloc(a.blk_max, (size_t)(std::max(g.n, g.nb) + 255) / 256 + 1));
+        TRY(alloc(a.skin_part, 8 * 16));
         if (!std::is_same<R, double>::value) {
             TRY(alloc(a.p4, n)); TRY(alloc(a.v4, n));
             TRY(alloc(a.wact, n - g.wb));
@@ -649,7 +649,7 @@ class Engine final : public dmph_handle {
     bool capturing = false;
     int enqueue_step() {
         timed(DMPH_TIMING_PREDICT, [&] { launch_predict<R>(g, a, stream); });
-        timed(DMPH_TIMING_REBIN, [&] { launch_rebin<R>(g, a, stream); });
+        timed(DMPH_TIMING_REBIN, [&] { launch_rebin<R>(g, a, stream, 1); });
         const bool branch = capturing && !std::is_same<R, double>::value && g.fused && !g.st_active &&
                             g.se > g.sb && !(getenv("DMPH_BRANCHES") && !strcmp(getenv("DMPH_BRANCHES"), "serial"));
         if (branch) {
