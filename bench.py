@@ -171,13 +171,13 @@ def kernel_traffic(kernel_name: str):
     return sum(hits) if hits else None
 
 
-def time_variant(D, sc, params, variant, steps, warmup):
+def time_variant(D, sc, params, variant, steps, warmup, cell_scale=-1.0):
     """Timed region: `steps` replays of the captured step (hipGraph), bracketed
     by device synchronisation.  Then the same number of steps launched one by
     one with HIP events around each kernel group, on the library's stream, for
-    the per-kernel durations.  FP32 variants tune the cell size at creation
-    (cell_scale < 0: each candidate width on a handle of its own)."""
-    s = D.Solver(D.Params.from_dict(params), sc, rigid=True, graph=True, cell_scale=-1.0, **VARIANTS[variant])
+    the per-kernel durations.  FP32 variants tune the cell size at creation by
+    default (cell_scale < 0: each candidate width on a handle of its own)."""
+    s = D.Solver(D.Params.from_dict(params), sc, rigid=True, graph=True, cell_scale=cell_scale, **VARIANTS[variant])
     info = s.info()
     s.step(warmup)
     s.sync()
@@ -422,6 +422,9 @@ def main(argv=None) -> int:
                     help="surrogate strides of the timed surrogate + fallback run (two re-anchor periods)")
     ap.add_argument("--no-channel", action="store_true", help="skip the large-channel (C5) measurement")
     ap.add_argument("--channel-steps", type=int, default=100)
+    ap.add_argument("--cell-scale", type=float, default=-1.0,
+                    help="cell width of the C2 handles over the reference's (default: tuned at creation over 1, "
+                         "1.25 and 1.5; a fixed value keeps the tuner's extra handles out of a profile)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="also run one device-to-device copy of MIB MiB (counter calibration)")
     a = ap.parse_args(argv)
@@ -439,11 +442,11 @@ def main(argv=None) -> int:
     n = sc.n
     dt = params["Dt"]
     print("bench: %d particles, headline variant %s" % (n, head), file=sys.stderr, flush=True)
-    mixed = time_variant(D, sc, params, head, a.steps, a.warmup)
+    mixed = time_variant(D, sc, params, head, a.steps, a.warmup, a.cell_scale)
     variants = {head: mixed}
     if not a.no_plain and prec == D.PREC_FP32:
         print("bench: plain FP32 (sums and rigid bodies in FP32)", file=sys.stderr, flush=True)
-        variants["plain"] = time_variant(D, sc, params, "plain", a.steps, a.warmup)
+        variants["plain"] = time_variant(D, sc, params, "plain", a.steps, a.warmup, a.cell_scale)
     if not a.no_fp64 and prec == D.PREC_FP32:
         print("bench: FP64 (the parity mode)", file=sys.stderr, flush=True)
         variants["fp64"] = time_variant(D, sc, params, "fp64", a.fp64_steps, a.warmup)
