@@ -111,6 +111,51 @@ VARIANTS = {
 }
 
 
+GPU_FIELDS = ("pos", "vel", "vol_strain_p", "pressure_p", "mass", "force")
+
+
+def gpu_state(s, lo=0, hi=None, b0=0, nb=None):
+    """The fields the errors compare, for particles [lo, hi) of a handle (one
+    scene of a batch) and its bodies [b0, b0 + nb), in original order."""
+    st = {f: s.field(f)[lo:hi] for f in GPU_FIELDS}
+    b = s.bodies()
+    st["body_x"] = b["x"][b0:None if nb is None else b0 + nb]
+    st["body_v"] = b["v"][b0:None if nb is None else b0 + nb]
+    return st
+
+
+def field_errors(st, o, sc, steps):
+    """Per-field errors of a GPU state (gpu_state) against the FP64 oracle at
+    the same step, plus mass and momentum balance of both."""
+    typ = o.field("type")
+    fl, so = typ < 2, (typ >= 2) & (typ < 4)
+    dx = st["pos"] - o.field("pos")
+    L = sc.domain_max - sc.domain_min
+    dx -= L * np.round(dx / L)
+    n0p = o.scalars()["n0p"]
+    nd, ndr = st["vol_strain_p"] + n0p, o.field("vol_strain_p") + n0p
+    m, mr = st["mass"], o.field("mass")
+    F, Fr = st["force"], o.field("force")
+    v, vr = st["vel"], o.field("vel")
+    bx, bv = o.get("body_x").reshape(-1, 3), o.get("body_v").reshape(-1, 3)
+    M, P = balance(m, v, typ)
+    Mr, Pr = balance(mr, vr, typ)
+    return {
+        "steps": steps,
+        "position_max_abs_m": float(np.abs(dx).max()),
+        "velocity_norm_rel": _nrel(v, vr),
+        "number_density_max_rel": float(np.max(np.abs(nd - ndr) / np.abs(ndr))),
+        "pressure_p_norm_rel": _nrel(st["pressure_p"][fl], o.field("pressure_p")[fl]),
+        "fluid_accel_norm_rel": _nrel(F[fl] / m[fl, None], Fr[fl] / mr[fl, None]),
+        "solid_force_norm_rel": _nrel(F[so], Fr[so]),
+        "body_position_max_abs_m": float(np.abs(st["body_x"] - bx).max()) if len(bx) else 0.0,
+        "body_velocity_norm_rel": _nrel(st["body_v"], bv) if len(bv) else 0.0,
+        "mass_total": M, "mass_total_oracle": Mr,
+        "momentum": [float(x) for x in P], "momentum_oracle": [float(x) for x in Pr],
+        "momentum_rel_diff": float(np.linalg.norm(P - Pr) / max(np.linalg.norm(Pr), 1e-300)),
+    }
+
+
 def errors_vs_oracle(D, sc, params, variant, o, steps, cell_scale=1.0):
     """Per-field errors of a fresh GPU run of `steps` steps (the variant's cell
     size) against the FP64 oracle at the same step, plus mass and momentum
@@ -118,35 +163,25 @@ def errors_vs_oracle(D, sc, params, variant, o, steps, cell_scale=1.0):
     s = D.Solver(D.Params.from_dict(params), sc, rigid=True, cell_scale=cell_scale, **VARIANTS[variant])
     s.step(steps)
     s.sync()
-    typ = o.field("type")
-    fl, so = typ < 2, (typ >= 2) & (typ < 4)
-    dx = s.field("pos") - o.field("pos")
-    L = sc.domain_max - sc.domain_min
-    dx -= L * np.round(dx / L)
-    n0p = o.scalars()["n0p"]
-    nd, ndr = s.field("vol_strain_p") + n0p, o.field("vol_strain_p") + n0p
-    m, mr = s.field("mass"), o.field("mass")
-    F, Fr = s.field("force"), o.field("force")
-    v, vr = s.field("vel"), o.field("vel")
-    b = s.bodies()
-    bx, bv = o.get("body_x").reshape(-1, 3), o.get("body_v").reshape(-1, 3)
-    M, P = balance(m, v, typ)
-    Mr, Pr = balance(mr, vr, typ)
-    out = {
-        "steps": steps,
-        "position_max_abs_m": float(np.abs(dx).max()),
-        "velocity_norm_rel": _nrel(v, vr),
-        "number_density_max_rel": float(np.max(np.abs(nd - ndr) / np.abs(ndr))),
-        "pressure_p_norm_rel": _nrel(s.field("pressure_p")[fl], o.field("pressure_p")[fl]),
-        "fluid_accel_norm_rel": _nrel(F[fl] / m[fl, None], Fr[fl] / mr[fl, None]),
-        "solid_force_norm_rel": _nrel(F[so], Fr[so]),
-        "body_position_max_abs_m": float(np.abs(b["x"] - bx).max()),
-        "body_velocity_norm_rel": _nrel(b["v"], bv),
-        "mass_total": M, "mass_total_oracle": Mr,
-        "momentum": [float(x) for x in P], "momentum_oracle": [float(x) for x in Pr],
-        "momentum_rel_diff": float(np.linalg.norm(P - Pr) / max(np.linalg.norm(Pr), 1e-300)),
-    }
+    out = field_errors(gpu_state(s), o, sc, steps)
     s.close()
+    return out
+
+
+def ensemble_errors_vs_oracle(scenes, snaps, params, steps):
+    """The checker for the batched ensemble: each sampled scene of the batched
+    mixed handle (its state after `steps` steps, taken from the batch) against
+    the FP64 oracle run on that scene alone for the same steps (its own skin
+    schedule; the batch rebins when any scene needs it, which moves only the
+    rebinning steps)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as P
+    out = {}
+    for k, st in snaps.items():
+        o = P.Oracle(P.params_from_dict(params), scenes[k], sort_mode=P.SORT_CANONICAL, rigid=True)
+        o.step(steps)
+        out["scene_%d" % k] = dict(field_errors(st, o, scenes[k], steps), particles=int(scenes[k].n))
+        o.close()
     return out
 
 
@@ -157,6 +192,26 @@ def copy_probe(mib: int) -> None:
 
 
 TRAFFIC_JSON = os.path.join(ROOT, "profiles", "dam_break_c2_fp32_traffic.json")
+
+
+def _params_c3():
+    from dem_mph_amd import scenes as S
+    return S.params_c3()
+
+
+def kernel_utilisation(kernel_name: str):
+    """VALU issue and vector-L1 lookup utilisation of a kernel from the
+    committed rocprofv3 counters (the same summary as the traffic), or None."""
+    try:
+        js = json.load(open(TRAFFIC_JSON))
+    except Exception:
+        return None
+    out = {}
+    for sec, keys in (("counters", ("valu_issue_util",)), ("l1", ("l1_lookup_util", "lookups_per_vmem"))):
+        for name, v in js.get(sec, {}).items():
+            if kernel_name in name:
+                out.update({k: v[k] for k in keys if k in v})
+    return out or None
 
 
 def kernel_traffic(kernel_name: str):
@@ -240,6 +295,8 @@ def ensemble_bench(D, nscenes: int, steps: int, warmup: int, seed: int, sample: 
                        "1-32 rigid cuboids of side 3-8, dt = 5e-5 s" % nscenes,
            "scenes": nscenes, "particles": n, "steps": steps, "generation_s": t_gen}
     res = {}
+    snaps = {}   # sampled scenes of the batched mixed handle after its last timed step, for the checker
+    sample_k = sorted({0, nscenes // 3, (2 * nscenes) // 3, nscenes - 1})
     cs = -1.0   # the mixed handle tunes the cell size at creation; the plain one reuses its choice
     for name, var in (("batched_mixed", "mixed"), ("batched_plain", "plain")):
         t0 = time.perf_counter()
@@ -255,6 +312,11 @@ def ensemble_bench(D, nscenes: int, steps: int, warmup: int, seed: int, sample: 
         s.step(steps)
         s.sync()
         el = time.perf_counter() - t0
+        if name == "batched_mixed":
+            off = s.scene_offsets
+            nbod = [int(sc.body.max()) + 1 if sc.body is not None and (sc.body >= 0).any() else 0 for sc in scenes]
+            for k in sample_k:
+                snaps[k] = gpu_state(s, off[k], off[k + 1], sum(nbod[:k]), nbod[k])
         s.close()
         res[name] = {"particle_steps_per_s": n * steps / el, "ms_per_step": el / steps * 1e3,
                      "sim_s_per_wall_s": dt * steps / el, "create_s": t_create, "autotuned_lanes": list(lanes),
@@ -288,6 +350,7 @@ def ensemble_bench(D, nscenes: int, steps: int, warmup: int, seed: int, sample: 
                                             res["sequential_plain"]["sim_s_per_wall_s"])
     out["batched_mixed_over_batched_plain"] = (res["batched_mixed"]["sim_s_per_wall_s"] /
                                                res["batched_plain"]["sim_s_per_wall_s"])
+    out["_snapshots"] = (snaps, warmup + 2 + steps)
     return out, scenes
 
 
@@ -510,6 +573,9 @@ def main(argv=None) -> int:
                                                     "per fluid or solid slot (sweep 2)": ALGO_BYTES["sweep2"]},
                      "units": {"particles": n, "fluid_and_solid_slots": fs},
                      "avg_ms_per_step": avg_s * 1e3,
+                     # what bounds the sweeps below the HBM roof: VALU issue and
+                     # vector-L1 tag lookups, from the same committed profile
+                     "utilisation": {k: kernel_utilisation(k) for k in knames},
                      "whole_step": {"algorithmic_bytes_per_particle": STEP_BYTES, "achieved": step_achieved,
                                     "frac": step_achieved / HBM_PEAK_GBS}},
     }
@@ -546,7 +612,12 @@ def main(argv=None) -> int:
         res = section("ensemble", lambda: ensemble_bench(D, a.ensemble_scenes, a.ensemble_steps, 5, a.seed,
                                                          a.ensemble_sample))
         ens, scenes = res if isinstance(res, tuple) else (res, None)
+        snaps = ens.pop("_snapshots", None)
         out["ensemble"] = ens
+        if snaps is not None and not a.no_cpu_baseline:
+            print("bench: ensemble scenes against the FP64 oracle", file=sys.stderr, flush=True)
+            ens["batched_mixed"]["errors_vs_fp64_oracle"] = section(
+                "ensemble_errors", lambda: ensemble_errors_vs_oracle(scenes, snaps[0], _params_c3(), snaps[1]))
         if not a.no_surrogate and scenes is not None:
             print("bench: learned surrogate", file=sys.stderr, flush=True)
             ens["surrogate"] = section("surrogate", lambda: surrogate_bench(
