@@ -357,12 +357,24 @@ def ensemble_bench(D, nscenes: int, steps: int, warmup: int, seed: int, sample: 
 SURROGATE_WEIGHTS = os.path.join(ROOT, "dem_mph_amd", "surrogate_c3.npz")
 
 
-def channel_bench(D, steps: int, warmup: int, seed: int, surrogate: bool):
+def judge_summary(j, tol):
+    """The judge's checks of a hybrid run against the true error at each check."""
+    return {"checks": len(j), "fell_back_checks": sum(1 for x in j if x["fell_back"]),
+            "true_error_m_mean": float(np.mean([x["true_error_m"] for x in j])) if j else None,
+            "estimate_m_mean": float(np.mean([x["estimate_m"] for x in j])) if j else None,
+            "kept_with_true_error_over_tol": sum(1 for x in j if not x["fell_back"] and x["true_error_m"] > tol),
+            "fell_with_true_error_within_tol": sum(1 for x in j if x["fell_back"] and x["true_error_m"] <= tol)}
+
+
+def channel_bench(D, steps: int, warmup: int, seed: int, surrogate: bool, config_steps: int = 1000):
     """The large single scene (C5 channel, 12.9M particles): the mixed solver
-    timed over `steps` replays of the captured step after autotuning, and the
-    learned surrogate's rollout throughput on the same particles (the network is
-    trained on C3 scenes; its accuracy on this geometry is not evaluated, so the
-    plain solver is the fallback that would take over)."""
+    timed over `steps` replays of the captured step after autotuning; the
+    learned surrogate's rollout throughput on the same particles; and the
+    surrogate with its plain-solver fallback run as deployed over the
+    configuration's `config_steps` steps from t = 0 (judged once, halfway),
+    with its debris-centroid error against the mixed-precision solver's
+    trajectory.  The network is trained on C3 scenes (d = 0.01 m, the C1 tank);
+    here it runs at this scene's spacing and in this channel."""
     from dem_mph_amd import scenes as S
     t0 = time.perf_counter()
     sc = S.channel_c5(seed=seed)
@@ -401,8 +413,21 @@ def channel_bench(D, steps: int, warmup: int, seed: int, surrogate: bool):
         out["surrogate"] = {"ms_per_step": t * 1e3, "stride_dt": SG.STRIDE, "finite": r["finite"],
                             "sim_s_per_wall_s": SG.STRIDE * dt / t, "particle_steps_per_s": n * SG.STRIDE / t,
                             "vs_mixed": (SG.STRIDE * dt / t) / (dt * steps / el),
-                            "accuracy": "not evaluated on this geometry (trained on C3 scenes)"}
+                            }
         print("bench: channel surrogate %.1f ms/step" % (t * 1e3), file=sys.stderr, flush=True)
+        if config_steps > 0:
+            # as deployed: solver warm-up, surrogate strides, one judgement by a
+            # solver probe halfway, fallback from the last anchored solver state
+            strides = config_steps // SG.STRIDE - (SG.HIST - 1)
+            tol = 0.01 * 1.6   # 1 % of the channel length
+            sur = SG.load(SURROGATE_WEIGHTS).at_spacing(sc.d).in_tank((0.0, 0.0, 0.0), (1.6, 0.8, 0.8))
+            print("bench: channel surrogate + fallback, %d strides" % strides, file=sys.stderr, flush=True)
+            hyb = SG.hybrid_report(sur, [sc], params, strides, reanchor=strides // 2, tol=tol)
+            j = hyb.pop("judgements")
+            hyb["judge"] = dict(judge_summary(j, tol), invalid_states=hyb.pop("invalid_states", 0))
+            out["surrogate"]["hybrid"] = hyb
+            print("bench: channel surrogate + fallback %.1f s (plain solver %.1f s)"
+                  % (hyb["seconds"], hyb["plain_batched_seconds"]), file=sys.stderr, flush=True)
     return out
 
 
@@ -433,12 +458,7 @@ def surrogate_bench(D, scenes, seq_ms_per_dt: float, batched_plain_ms_per_dt: fl
     hyb = SG.hybrid_report(sur, ev, params, hybrid_strides)
     hyb.pop("judgements_detail", None)
     j = hyb.pop("judgements")
-    hyb["judge"] = {"checks": len(j), "fell_back_checks": sum(1 for x in j if x["fell_back"]),
-                    "invalid_states": hyb.pop("invalid_states", 0),
-                    "true_error_m_mean": float(np.mean([x["true_error_m"] for x in j])) if j else None,
-                    "estimate_m_mean": float(np.mean([x["estimate_m"] for x in j])) if j else None,
-                    "kept_with_true_error_over_tol": sum(1 for x in j if not x["fell_back"] and x["true_error_m"] > 0.008),
-                    "fell_with_true_error_within_tol": sum(1 for x in j if x["fell_back"] and x["true_error_m"] <= 0.008)}
+    hyb["judge"] = dict(judge_summary(j, 0.008), invalid_states=hyb.pop("invalid_states", 0))
     t_step = 0.0
     finite = True
     nch = (len(scenes) + chunk - 1) // chunk   # equal chunks of at most `chunk` scenes
@@ -485,6 +505,8 @@ def main(argv=None) -> int:
                     help="surrogate strides of the timed surrogate + fallback run (two re-anchor periods)")
     ap.add_argument("--no-channel", action="store_true", help="skip the large-channel (C5) measurement")
     ap.add_argument("--channel-steps", type=int, default=100)
+    ap.add_argument("--channel-config-steps", type=int, default=1000,
+                    help="steps of the channel's surrogate-with-fallback run from t = 0 (0: skip)")
     ap.add_argument("--cell-scale", type=float, default=-1.0,
                     help="cell width of the C2 handles over the reference's (default: tuned at creation over 1, "
                          "1.25 and 1.5; a fixed value keeps the tuner's extra handles out of a profile)")
@@ -626,7 +648,8 @@ def main(argv=None) -> int:
         del scenes
     if not a.no_channel:
         print("bench: large channel", file=sys.stderr, flush=True)
-        out["channel"] = section("channel", lambda: channel_bench(D, a.channel_steps, 5, a.seed, not a.no_surrogate))
+        out["channel"] = section("channel", lambda: channel_bench(D, a.channel_steps, 5, a.seed, not a.no_surrogate,
+                                                                  a.channel_config_steps))
     if failed:
         out["failed_sections"] = failed
     print(json.dumps(out))

@@ -171,6 +171,13 @@ class Surrogate:
         self._bbox = None
         return self
 
+    def in_tank(self, lo, hi) -> "Surrogate":
+        """Use the network in a tank whose interior is the box [lo, hi]: the
+        clipped wall distances of the node features are taken from its faces."""
+        self.tank_lo = np.asarray(lo, float)
+        self.tank_hi = np.asarray(hi, float)
+        return self
+
     # ------------------------------------------------------------------ parameters
     def _refresh_bf16(self) -> None:
         """bf16 copies of every GEMM weight (an Adam step with lr 0 rewrites them)."""
