@@ -459,6 +459,13 @@ def surrogate_bench(D, scenes, seq_ms_per_dt: float, batched_plain_ms_per_dt: fl
     hyb.pop("judgements_detail", None)
     j = hyb.pop("judgements")
     hyb["judge"] = dict(judge_summary(j, 0.008), invalid_states=hyb.pop("invalid_states", 0))
+    # the scene-by-scene plain solver on these scenes, estimated from its
+    # measured cost per particle-step on the ensemble (sequential_plain)
+    n_ev = sum(sc.n for sc in ev)
+    n_all = sum(sc.n for sc in scenes)
+    seq_s = seq_ms_per_dt * 1e-3 * (n_ev / n_all) * hyb["simulated_s"] / dt
+    hyb["sequential_plain_seconds_estimate"] = seq_s
+    hyb["vs_sequential_plain_estimate"] = seq_s / hyb["seconds"]
     t_step = 0.0
     finite = True
     nch = (len(scenes) + chunk - 1) // chunk   # equal chunks of at most `chunk` scenes
