@@ -72,6 +72,8 @@ struct Scalars {
     unsigned int ticket[4];         // blocks finished, per kernel that ends in a last-block phase
     int full_sweep;                 // FP32: > 0 -> sweep 1 covers every wall particle (the key pass counts it down)
     int walls_moving;               // this step moved the walls (0.2 <= Time <= 0.4, written by k_predict)
+    int walls_dirty;                // walls moved (or the host set positions) since the last rebinning
+    int build_walls_dirty;          // ... as of this rebinning: the list build re-walks every wall row
 };
 
 template <class R>
