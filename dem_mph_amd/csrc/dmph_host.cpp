@@ -548,7 +548,7 @@ class Engine final : public dmph_handle {
         TRY(alloc(a.b_q, 4 * nb)); TRY(alloc(a.b_iinv, 9 * nb));
         TRY(alloc(a.b_w, 3 * nb)); TRY(alloc(a.b_r, 9 * nb));
         // member map: each member's rank in its body (input order), the body's
-        // offsets; k_gather fills the slots of the ranks after every rebinning
+        // offsets; the rebinning (k_place or k_gather) fills the slots of the ranks
         std::vector<int> mrank(n, -1), moff(nb + 1, 0);
         g.nloose = 0;
         for (int i = 0; i < n; i++) {
