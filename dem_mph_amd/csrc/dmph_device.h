@@ -36,6 +36,7 @@ struct Geo {
     int rigid;          // solid particles with body >= 0 move as rigid bodies
     int sort_reference; // within-cell order from the reference's bitonic network
     int rebuild_every;  // rebin every step instead of the skin test
+    int stage_b;        // FP32 pass B with the neighbour cells staged in LDS (k_pass_b_lds)
     int p2, p2_log;     // bitonic size (power of two > n) and its log2
     int compensated;    // FP32 mode: FP64 accumulators for the pair sums
     int kcap;           // neighbour-list capacity per particle

@@ -295,6 +295,7 @@ class Engine final : public dmph_handle {
         g.body_f32 = o.rigid_precision == DMPH_PREC_FP32;
         g.fuse_accel = 1;
         g.fused = getenv("DMPH_SWEEPS") && !strcmp(getenv("DMPH_SWEEPS"), "separate") ? 0 : 1;
+        g.stage_b = getenv("DMPH_PASS_B") && !strcmp(getenv("DMPH_PASS_B"), "lds") ? 1 : 0;
         g.sort_reference = o.sort_mode == DMPH_SORT_REFERENCE;
         g.rebuild_every = o.rebuild == DMPH_REBUILD_EVERY_STEP;
         weights(g, p);

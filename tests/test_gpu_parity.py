@@ -398,6 +398,29 @@ def test_static_wall_dem_coefficients_kept():
     a.close(); b.close()
 
 
+@pytest.mark.parametrize("lanes", [2, 4, 8])
+def test_pass_b_staged_in_lds_equals_global_gathers(monkeypatch, lanes):
+    """Pass B with its neighbour cells staged in LDS (DMPH_PASS_B=lds: a
+    workgroup copies the 27 slot runs around a tile of cells, then gathers from
+    LDS) walks the same entries in the same order with the same arithmetic as
+    the default pass B: after 40 steps of C1 at full size (rebuilds included)
+    and of mix3d, every field equals the default run's bit for bit (tiles at
+    the grid's ends, and tiles with more records than the buffer, take the
+    kernel's global-gather fallback)."""
+    for sc, prm in ((S.dam_break_c1(seed=3, rigid=True), S.params_c1()),
+                    (G.scene(G.load("mix3d")), G.params(G.load("mix3d")))):
+        out = []
+        for mode in ("", "lds"):
+            monkeypatch.setenv("DMPH_PASS_B", mode)
+            s = D.Solver(D.Params.from_dict(prm), sc, precision=D.PREC_FP32, rigid=sc.body is not None, lanes=lanes)
+            s.step(40)
+            s.sync()
+            out.append({f: s.field(f) for f in ("pos", "vel", "force", "pressure_p", "vol_strain_p")})
+            s.close()
+        for f in out[0]:
+            np.testing.assert_array_equal(out[1][f], out[0][f], err_msg=f)
+
+
 def test_cell_size_tuned_at_creation():
     """cell_scale < 0 tunes the cell width at creation (FP32): the handle comes
     back on one of the candidate grids (1, 1.25 or 1.5 x the reference's cell),
