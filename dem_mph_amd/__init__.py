@@ -36,7 +36,7 @@ FIELDS = {
 
 EXPORTED = ("dmph_default_options", "dmph_params_read", "dmph_grid_read", "dmph_grid_free", "dmph_create",
             "dmph_create_batch", "dmph_step", "dmph_sync", "dmph_time", "dmph_get_field", "dmph_set_field",
-            "dmph_get_bodies", "dmph_get_info",
+            "dmph_get_bodies", "dmph_get_info", "dmph_count_nonfinite",
             "dmph_virial", "dmph_pairs", "dmph_write_prof", "dmph_write_vtk", "dmph_set_timing", "dmph_get_timing",
             "dmph_autotune", "dmph_export_state",
             "dmph_boid_generate", "dmph_boid_error", "dmph_grid_write",
@@ -164,6 +164,7 @@ def lib() -> C.CDLL:
         L.dmph_get_bodies.argtypes = [vp, dp, dp, dp, dp, dp]
         L.dmph_get_info.argtypes = [vp, C.POINTER(Info)]
         L.dmph_virial.argtypes = [vp]
+        L.dmph_count_nonfinite.argtypes = [vp, C.c_int, C.POINTER(C.c_int64)]
         L.dmph_pairs.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64), ip]
         L.dmph_write_prof.argtypes = [vp, C.c_char_p]
         L.dmph_write_vtk.argtypes = [vp, C.c_char_p]
@@ -343,6 +344,12 @@ class Solver:
 
     def virial(self) -> None:
         self._check(lib().dmph_virial(self.h), "dmph_virial")
+
+    def count_nonfinite(self, name: str) -> int:
+        """NaN or infinite values of a field, counted on the device (dmph_count_nonfinite)."""
+        c = C.c_int64()
+        self._check(lib().dmph_count_nonfinite(self.h, FIELDS[name][0], C.byref(c)), "dmph_count_nonfinite")
+        return int(c.value)
 
     def pairs(self):
         """(ptr[n+1], idx[nnz]) of the neighbour list at the last rebinning, in slot order."""

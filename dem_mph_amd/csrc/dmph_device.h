@@ -106,6 +106,7 @@ struct Dev {
     double *b_mass, *b_x, *b_v, *b_l, *b_q, *b_iinv, *b_w, *b_r;
     int *mrank;                    // [n] rank of a member within its body (fixed at set-up, moves with the particle)
     int *b_moff, *b_mslot;         // per body: offsets into b_mslot [nb + 1]; member slots [members], by rank
+    unsigned long long *nonfinite; // [1] counter of dmph_count_nonfinite
     unsigned long long *skin_part; // [8 x 16] max squared displacement this step, one slot per XCD (bits of a
                                    // non-negative double, 128 B apart), zeroed by the scan after the key pass reads it
     float4 *p4, *v4;               // FP32 sweeps: packed (position, type) and (velocity, PressureP)
@@ -131,6 +132,9 @@ template <class R> void launch_move(const Geo &g, const Dev<R> &a, hipStream_t s
 template <class R> void launch_virial(const Geo &g, const Dev<R> &a, hipStream_t s);
 template <class R> void launch_init_density(const Geo &g, const Dev<R> &a, hipStream_t s);
 template <class R> void launch_export(const Geo &g, const Dev<R> &a, float *pos, float *vel, hipStream_t s);
+// count of the non-finite values of x[0..n) (and y, z when not null) added to *count
+template <class R>
+void launch_count_nonfinite(const R *x, const R *y, const R *z, int n, unsigned long long *count, hipStream_t s);
 
 // FP32 sweeps (dmph_sweep32.inc, in the FP32 object only)
 void launch_sweep1_f32(const Geo &g, const Dev<float> &a, hipStream_t s, int init, hipStream_t s_solid = nullptr);

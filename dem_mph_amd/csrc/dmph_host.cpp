@@ -33,6 +33,7 @@ struct dmph_handle {
     virtual int get_bodies(double *x, double *v, double *q, double *l, double *m) = 0;
     virtual int info(dmph_info *out) = 0;
     virtual int virial() = 0;
+    virtual int count_nonfinite(int field, int64_t *count) = 0;
     virtual int pairs(int64_t *nnz, int64_t *ptr, int *idx) = 0;
     virtual int write_prof(const char *path) = 0;
     virtual int write_vtk(const char *path) = 0;
@@ -419,6 +420,7 @@ class Engine final : public dmph_handle {
         TRY(alloc(a.t_dbl, g.rigid ? 3 * n : 1)); TRY(alloc(a.r0, g.rigid ? 3 * n : 1));
         TRY(alloc(a.sc, 1));
         TRY(alloc(a.skin_part, 8 * 16));
+        TRY(alloc(a.nonfinite, 1));
         if (!std::is_same<R, double>::value) {
             TRY(alloc(a.p4, n)); TRY(alloc(a.v4, n));
             TRY(alloc(a.wact, n - g.wb));
@@ -1017,6 +1019,44 @@ class Engine final : public dmph_handle {
         return DMPH_OK;
     }
 
+    int count_nonfinite(int field, int64_t *count) override {
+        HCHK(hipSetDevice(device));
+        const R *p[3] = {nullptr, nullptr, nullptr};
+        switch (field) {
+            case DMPH_FIELD_POSITION: p[0] = a.px; p[1] = a.py; p[2] = a.pz; break;
+            case DMPH_FIELD_VELOCITY: p[0] = a.vx; p[1] = a.vy; p[2] = a.vz; break;
+            case DMPH_FIELD_FORCE: p[0] = a.fx; p[1] = a.fy; p[2] = a.fz; break;
+            case DMPH_FIELD_GRAVITY_CENTER: p[0] = a.gcx; p[1] = a.gcy; p[2] = a.gcz; break;
+            case DMPH_FIELD_DENSITY_A: p[0] = a.dens_a; break;
+            case DMPH_FIELD_PRESSURE_A: p[0] = a.press_a; break;
+            case DMPH_FIELD_VOL_STRAIN_P: p[0] = a.vs_p; break;
+            case DMPH_FIELD_DIVERGENCE_P: p[0] = a.div_p; break;
+            case DMPH_FIELD_PRESSURE_P: p[0] = a.press_p; break;
+            case DMPH_FIELD_MASS: p[0] = a.mass; break;
+            case DMPH_FIELD_ALPHA: p[0] = a.alpha; break;
+            case DMPH_FIELD_DRAG_C: p[0] = a.drag_c; break;
+            case DMPH_FIELD_KN: p[0] = a.kn; break;
+            case DMPH_FIELD_ETAN: p[0] = a.etan; break;
+            case DMPH_FIELD_KT: p[0] = a.kt; break;
+            case DMPH_FIELD_ETAT: p[0] = a.etat; break;
+            case DMPH_FIELD_VIRIAL_PRESSURE: p[0] = a.virial_p; break;
+            case DMPH_FIELD_KAPPA: p[0] = a.kappa; break;
+            case DMPH_FIELD_TYPE: case DMPH_FIELD_ID: case DMPH_FIELD_NEIGHBOR_COUNT:
+            case DMPH_FIELD_SOLID_NEIGHBOR_COUNT: case DMPH_FIELD_CELL_KEY: case DMPH_FIELD_BODY:
+                *count = 0;
+                return DMPH_OK;
+            default: err = "unknown field"; return DMPH_E_ARG;
+        }
+        HCHK(hipMemsetAsync(a.nonfinite, 0, sizeof(unsigned long long), stream));
+        launch_count_nonfinite<R>(p[0], p[1], p[2], g.n, a.nonfinite, stream);
+        HCHK(hipGetLastError());
+        unsigned long long c = 0;
+        HCHK(hipMemcpyAsync(&c, a.nonfinite, sizeof c, hipMemcpyDeviceToHost, stream));
+        HCHK(hipStreamSynchronize(stream));
+        *count = (int64_t)c;
+        return DMPH_OK;
+    }
+
     int virial() override {
         HCHK(hipSetDevice(device));
         launch_virial<R>(g, a, stream);
@@ -1282,6 +1322,9 @@ int dmph_get_bodies(dmph_handle *h, double *x, double *v, double *q, double *l, 
 }
 int dmph_get_info(dmph_handle *h, dmph_info *info) { return h && info ? h->info(info) : DMPH_E_ARG; }
 int dmph_virial(dmph_handle *h) { return h ? h->virial() : DMPH_E_ARG; }
+int dmph_count_nonfinite(dmph_handle *h, int field, int64_t *count) {
+    return h && count ? h->count_nonfinite(field, count) : DMPH_E_ARG;
+}
 int dmph_pairs(dmph_handle *h, int64_t *nnz, int64_t *ptr, int *idx) { return h ? h->pairs(nnz, ptr, idx) : DMPH_E_ARG; }
 int dmph_write_prof(dmph_handle *h, const char *path) { return h && path ? h->write_prof(path) : DMPH_E_ARG; }
 int dmph_write_vtk(dmph_handle *h, const char *path) { return h && path ? h->write_vtk(path) : DMPH_E_ARG; }

@@ -212,6 +212,14 @@ typedef struct dmph_info {
 } dmph_info;
 int dmph_get_info(dmph_handle *h, dmph_info *info);
 
+/* The optional NaN/Inf check of the boundary: counts, on the device, the
+ * values of a real field (DMPH_FIELD_* other than the integer ones, every
+ * component) that are NaN or infinite, into *count (0: all finite; integer
+ * fields count 0).  The reference lets NaN propagate silently (the drag at zero
+ * relative velocity, source/DEM-MPH.cpp:2775-2797, divides by Re = 0); this
+ * lets a caller stop a run that produced one.  Synchronous. */
+int dmph_count_nonfinite(dmph_handle *h, int field, int64_t *count);
+
 /* calculateVirialStressAtParticle (source/DEM-MPH.cpp:2904-3149), the
  * output-only pass run before each VTK write. */
 int dmph_virial(dmph_handle *h);

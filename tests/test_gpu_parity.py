@@ -610,6 +610,28 @@ def test_set_field_positions_and_velocities(prec):
     assert agree == 1.0 if prec == D.PREC_FP64 else agree > 0.999
 
 
+@pytest.mark.parametrize("prec", [D.PREC_FP64, D.PREC_FP32])
+def test_count_nonfinite(prec):
+    """dmph_count_nonfinite counts, on the device, the NaN and infinite values
+    of a field: none in the state of a normal run (mix3d, 3 steps); exactly the
+    three velocity components written as NaN, +inf and -inf through
+    dmph_set_field (two of one particle, one of another), while positions stay
+    clean; integer fields count 0.  The handle is closed without stepping."""
+    g = G.load("mix3d")
+    s = _solver(g, precision=prec)
+    s.step(3)
+    s.sync()
+    for f in ("pos", "vel", "pressure_p", "vol_strain_p", "mass"):
+        assert s.count_nonfinite(f) == 0, f
+    v = s.field("vel")
+    v[5, 0], v[5, 2], v[17, 1] = np.nan, np.inf, -np.inf
+    s.set_field("vel", v)
+    assert s.count_nonfinite("vel") == 3
+    assert s.count_nonfinite("pos") == 0
+    assert s.count_nonfinite("type") == 0
+    s.close()
+
+
 def test_autotune_keeps_state_and_tolerance():
     """dmph_autotune picks 2, 4, 8 or 16 lanes per sweep without advancing or
     changing the state; the next steps stay within the FP32 tolerances."""
