@@ -230,8 +230,7 @@ class Engine final : public dmph_handle {
 
     ~Engine() override {
         if (stream) (void)hipStreamSynchronize(stream);
-        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
-        if (graph) (void)hipGraphDestroy(graph);
+        drop_graph();
         for (auto &p_ : pending) { ev_pool.push_back(p_.second.first); ev_pool.push_back(p_.second.second); }
         for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
         for (void *q : allocs) (void)hipFree(q);
@@ -683,17 +682,24 @@ class Engine final : public dmph_handle {
     }
 
     // The step as a HIP graph, captured on first use and replayed once per step.
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t graph_exec = nullptr;
+    // One step, and kGraphSteps steps in a row: dmph_step(n) replays the long
+    // graph while n allows, so consecutive steps run without a graph launch
+    // between them (each launch left the GPU idle ~10 us between steps).
+    static constexpr int kGraphSteps = 10;
+    hipGraph_t graph = nullptr, graph_k = nullptr;
+    hipGraphExec_t graph_exec = nullptr, graph_exec_k = nullptr;
     void drop_graph() {
-        if (graph_exec) { (void)hipGraphExecDestroy(graph_exec); graph_exec = nullptr; }
-        if (graph) { (void)hipGraphDestroy(graph); graph = nullptr; }
+        for (hipGraphExec_t *x : {&graph_exec, &graph_exec_k})
+            if (*x) { (void)hipGraphExecDestroy(*x); *x = nullptr; }
+        for (hipGraph_t *x : {&graph, &graph_k})
+            if (*x) { (void)hipGraphDestroy(*x); *x = nullptr; }
     }
 
-    int capture_step() {
+    int capture_steps(int k, hipGraph_t &graph_out, hipGraphExec_t &exec_out) {
         HCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
         capturing = true;
-        const int rc = enqueue_step();
+        int rc = DMPH_OK;
+        for (int s = 0; s < k && rc == DMPH_OK; s++) rc = enqueue_step();
         capturing = false;
         hipGraph_t gr = nullptr;
         const hipError_t e = hipStreamEndCapture(stream, &gr);
@@ -705,8 +711,8 @@ class Engine final : public dmph_handle {
             err = std::string("hipStreamEndCapture: ") + hipGetErrorString(e);
             return DMPH_E_DEVICE;
         }
-        graph = gr;
-        HCHK(hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0));
+        graph_out = gr;
+        HCHK(hipGraphInstantiate(&exec_out, graph_out, nullptr, nullptr, 0));
         return DMPH_OK;
     }
 
@@ -772,8 +778,7 @@ class Engine final : public dmph_handle {
         HCHK(hipStreamSynchronize(stream));
         HCHK(hipGetLastError());
         // launch arguments changed: capture the step again at the next call
-        if (graph_exec) { (void)hipGraphExecDestroy(graph_exec); graph_exec = nullptr; }
-        if (graph) { (void)hipGraphDestroy(graph); graph = nullptr; }
+        drop_graph();
         if (lanes) { lanes[0] = g.lanes[0]; lanes[1] = g.lanes[1]; }
         return DMPH_OK;
     }
@@ -783,11 +788,14 @@ class Engine final : public dmph_handle {
         HCHK(hipSetDevice(device));
         // per-group event timing needs the launches one by one: eager then
         const bool use_graph = o.graph && !timing;
-        if (use_graph && !graph_exec && nsteps > 0) TRY(capture_step());
-        for (int k = 0; k < nsteps; k++) {
-            if (use_graph) {
-                HCHK(hipGraphLaunch(graph_exec, stream));
-            } else {
+        if (use_graph) {
+            int left = nsteps;
+            if (left >= kGraphSteps && !graph_exec_k) TRY(capture_steps(kGraphSteps, graph_k, graph_exec_k));
+            for (; left >= kGraphSteps; left -= kGraphSteps) HCHK(hipGraphLaunch(graph_exec_k, stream));
+            if (left > 0 && !graph_exec) TRY(capture_steps(1, graph, graph_exec));
+            for (; left > 0; left--) HCHK(hipGraphLaunch(graph_exec, stream));
+        } else {
+            for (int k = 0; k < nsteps; k++) {
                 TRY(enqueue_step());
                 if (timing && pending.size() > 4096) TRY(resolve_timing());
             }
