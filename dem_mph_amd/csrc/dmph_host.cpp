@@ -789,10 +789,13 @@ class Engine final : public dmph_handle {
         // per-group event timing needs the launches one by one: eager then
         const bool use_graph = o.graph && !timing;
         if (use_graph) {
+            // both graphs at the first call, whatever its count, so that a later
+            // call of another length replays without capturing (capture only
+            // records: nothing runs until a launch)
+            if (nsteps > 0 && !graph_exec) TRY(capture_steps(1, graph, graph_exec));
+            if (nsteps > 0 && !graph_exec_k) TRY(capture_steps(kGraphSteps, graph_k, graph_exec_k));
             int left = nsteps;
-            if (left >= kGraphSteps && !graph_exec_k) TRY(capture_steps(kGraphSteps, graph_k, graph_exec_k));
             for (; left >= kGraphSteps; left -= kGraphSteps) HCHK(hipGraphLaunch(graph_exec_k, stream));
-            if (left > 0 && !graph_exec) TRY(capture_steps(1, graph, graph_exec));
             for (; left > 0; left--) HCHK(hipGraphLaunch(graph_exec, stream));
         } else {
             for (int k = 0; k < nsteps; k++) {
