@@ -421,7 +421,7 @@ class Engine final : public dmph_handle {
         TRY(alloc(a.skin_part, 8 * 16));
         TRY(alloc(a.nonfinite, 1));
         if (!std::is_same<R, double>::value) {
-            TRY(alloc(a.p4, n)); TRY(alloc(a.v4, n));
+            TRY(alloc(a.p4, n + 4)); TRY(alloc(a.v4, n));   // the list build reads up to 3 slots past a cell
             TRY(alloc(a.wact, n - g.wb));
             TRY(alloc(a.wnfs, n - g.wb));
             TRY(alloc(a.wsq, n - g.wb));
