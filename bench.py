@@ -253,9 +253,10 @@ def time_variant(D, sc, params, variant, steps, warmup, cell_scale=-1.0):
     tm = s.timing()
     s.set_timing(False)
     blocks = list(s.info().block)
+    wall_lanes = int(s.info().wall_lanes)
     s.close()
     return {"seconds": el, "timing": tm, "info": info, "rebuilds": int(reb - reb0), "seconds_timed_launches": el_ev,
-            "lanes": list(lanes), "blocks": blocks, "cell_scale": float(info.cell_scale)}
+            "lanes": list(lanes), "blocks": blocks, "wall_lanes": wall_lanes, "cell_scale": float(info.cell_scale)}
 
 
 def full_config(D, sc, params, variant, steps, cell_scale=1.0):
@@ -317,10 +318,11 @@ def ensemble_bench(D, nscenes: int, steps: int, warmup: int, seed: int, sample: 
             nbod = [int(sc.body.max()) + 1 if sc.body is not None and (sc.body >= 0).any() else 0 for sc in scenes]
             for k in sample_k:
                 snaps[k] = gpu_state(s, off[k], off[k + 1], sum(nbod[:k]), nbod[k])
+        wall_lanes = int(s.info().wall_lanes)
         s.close()
         res[name] = {"particle_steps_per_s": n * steps / el, "ms_per_step": el / steps * 1e3,
                      "sim_s_per_wall_s": dt * steps / el, "create_s": t_create, "autotuned_lanes": list(lanes),
-                     "cell_scale": cs}
+                     "autotuned_wall_lanes": wall_lanes, "cell_scale": cs}
         print("bench: ensemble %s %.2f ms/step" % (name, el / steps * 1e3), file=sys.stderr, flush=True)
     # plain FP32 solver, one scene per handle, scenes one after another (each autotuned)
     idx = np.linspace(0, nscenes - 1, min(sample, nscenes)).round().astype(int)
@@ -591,7 +593,7 @@ def main(argv=None) -> int:
                          "steps": a.fp64_steps if k == "fp64" else a.steps,
                          "options": {kk: vv for kk, vv in VARIANTS[k].items()},
                          "autotuned_lanes": v["lanes"], "autotuned_blocks": v["blocks"],
-                         "autotuned_cell_scale": v["cell_scale"]}
+                         "autotuned_wall_lanes": v["wall_lanes"], "autotuned_cell_scale": v["cell_scale"]}
                      for k, v in variants.items()},
         "roofline": {"bound": "hbm", "kernel": "sweeps: pass A (%s, %s) + pass B (%s)" % tuple(knames),
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

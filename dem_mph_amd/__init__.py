@@ -111,7 +111,8 @@ class Grid(C.Structure):
 class Options(C.Structure):
     _fields_ = [("dim", C.c_int), ("precision", C.c_int), ("sort_mode", C.c_int), ("rebuild", C.c_int),
                 ("debris", C.c_int), ("device", C.c_int), ("compensated", C.c_int), ("graph", C.c_int),
-                ("lanes", C.c_int), ("rigid_precision", C.c_int), ("block", C.c_int), ("cell_scale", C.c_double)]
+                ("lanes", C.c_int), ("rigid_precision", C.c_int), ("block", C.c_int), ("cell_scale", C.c_double),
+                ("wall_lanes", C.c_int)]
 
 
 class Info(C.Structure):
@@ -120,7 +121,8 @@ class Info(C.Structure):
                 ("fluid_begin", C.c_int), ("fluid_end", C.c_int), ("solid_begin", C.c_int), ("solid_end", C.c_int),
                 ("wall_begin", C.c_int), ("wall_end", C.c_int), ("cell_width", C.c_double * 3),
                 ("n0a", C.c_double), ("n0p", C.c_double), ("max_radius", C.c_double), ("rebuilds", C.c_int64),
-                ("nscenes", C.c_int), ("lanes", C.c_int * 2), ("block", C.c_int * 2), ("cell_scale", C.c_double)]
+                ("nscenes", C.c_int), ("lanes", C.c_int * 2), ("block", C.c_int * 2), ("cell_scale", C.c_double),
+                ("wall_lanes", C.c_int)]
 
 
 _lib = None
@@ -243,7 +245,7 @@ class Solver:
     def __init__(self, params: Params, scene, precision: int = PREC_FP64, sort_mode: int = SORT_CANONICAL,
                  rebuild: int = REBUILD_SKIN, rigid: bool = False, dim: Optional[int] = None, device: int = 0,
                  compensated: bool = True, graph: bool = True, lanes: int = 0, rigid_precision: int = PREC_FP64,
-                 block: int = 0, cell_scale: float = 1.0):
+                 block: int = 0, cell_scale: float = 1.0, wall_lanes: int = 0):
         self._arrays = []
         scenes = list(scene) if isinstance(scene, (list, tuple)) else [scene]
         grids = (Grid * len(scenes))()
@@ -254,7 +256,8 @@ class Solver:
                                        precision=precision, sort_mode=sort_mode, rebuild=rebuild,
                                        debris=DEBRIS_RIGID if rigid else DEBRIS_PARTICLES, device=device,
                                        compensated=1 if compensated else 0, graph=1 if graph else 0, lanes=lanes,
-                                       rigid_precision=rigid_precision, block=block, cell_scale=cell_scale)
+                                       rigid_precision=rigid_precision, block=block, cell_scale=cell_scale,
+                                       wall_lanes=wall_lanes)
         self.params = params
         h = C.c_void_p()
         if len(scenes) == 1:

@@ -42,6 +42,7 @@ struct Geo {
     int kcap;           // neighbour-list capacity per particle
     int knear;          // FP32 rows: near segment capacity (far entries start at this position), a multiple of 16
     int lanes[2];       // FP32 sweeps 1 and 2: lanes per particle (2, 4, 8 or 16)
+    int lanes_w;        // FP32 pass A: lanes per wall row (1, 2 or 4)
     int block[2];       // FP32 sweeps 1 and 2: threads per workgroup (64, 128 or 256)
     int fuse_accel;     // FP32: the fluid sweep 2 also applies calculateAcceleration (0 while re-running sweeps)
     int fused;          // FP32 without surface tension: passes A and B in place of sweeps 1 and 2

@@ -291,6 +291,7 @@ class Engine final : public dmph_handle {
         g.substeps = (int)(p.dt / p.dem_dt);   // source/DEM-MPH.cpp:434
         g.compensated = o.compensated;
         g.lanes[0] = g.lanes[1] = (o.lanes == 2 || o.lanes == 4 || o.lanes == 16) ? o.lanes : 8;
+        g.lanes_w = (o.wall_lanes == 1 || o.wall_lanes == 4) ? o.wall_lanes : 2;
         g.block[0] = g.block[1] = (o.block == 64 || o.block == 128) ? o.block : 256;
         g.body_f32 = o.rigid_precision == DMPH_PREC_FP32;
         g.fuse_accel = 1;
@@ -760,6 +761,31 @@ class Engine final : public dmph_handle {
             g.lanes[w] = orig[w];
             g.block[w] = borig[w];
         }
+        // lanes per wall row in pass A (the fused passes' wall rows walk ~30
+        // entries between full sweeps, a fluid row ~140), at the lane count and
+        // workgroup size just picked for the fluid rows
+        const int worig = g.lanes_w;
+        int wpick = worig;
+        if (g.fused && !g.st_active && g.we > g.wb) {
+            const int wcand[3] = {1, 2, 4};
+            float best = 1e30f;
+            g.lanes[0] = picks[0];
+            g.block[0] = bpicks[0];
+            for (int c = 0; c < 3; c++) {
+                g.lanes_w = wcand[c];
+                launch_sweep1<R>(g, a, stream);
+                HCHK(hipEventRecord(e0, stream));
+                for (int r = 0; r < reps; r++) launch_sweep1<R>(g, a, stream);
+                HCHK(hipEventRecord(e1, stream));
+                HCHK(hipEventSynchronize(e1));
+                float ms = 0.f;
+                HCHK(hipEventElapsedTime(&ms, e0, e1));
+                if (ms < best) { best = ms; wpick = wcand[c]; }
+            }
+            g.lanes[0] = orig[0];
+            g.block[0] = borig[0];
+            g.lanes_w = worig;
+        }
         // the sweeps' outputs as the state's own lane counts computed them (the
         // candidates sum in other orders), then the forces as the DEM left them
         launch_sweep1<R>(g, a, stream);
@@ -768,6 +794,7 @@ class Engine final : public dmph_handle {
         g.lanes[1] = picks[1];
         g.block[0] = bpicks[0];
         g.block[1] = bpicks[1];
+        g.lanes_w = wpick;
         ev_pool.push_back(e0);
         ev_pool.push_back(e1);
         HCHK(hipMemcpyAsync(a.fx, save, n * sizeof(R), hipMemcpyDeviceToDevice, stream));
@@ -1023,6 +1050,7 @@ class Engine final : public dmph_handle {
         out->block[0] = std::is_same<R, double>::value ? 256 : g.block[0];
         out->block[1] = std::is_same<R, double>::value ? 256 : g.block[1];
         out->cell_scale = cell_scale;
+        out->wall_lanes = std::is_same<R, double>::value ? 1 : g.lanes_w;
         Scalars sc;
         HCHK(hipMemcpyAsync(&sc, a.sc, sizeof sc, hipMemcpyDeviceToHost, stream));
         HCHK(hipStreamSynchronize(stream));

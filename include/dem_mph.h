@@ -85,6 +85,7 @@ typedef struct dmph_options {
                            handle of their own (autotuned, timed between rebuilds and on rebuild steps, scored
                            as step time + rebuild cost over 20 steps) and the handle is created with the
                            best; dmph_get_info reports it (FP64: the reference's grid) */
+    int wall_lanes;     /* FP32 only: lanes per wall row in the fused pass A, 1, 2 or 4 (0: 2; see dmph_autotune) */
 } dmph_options;
 
 void dmph_default_options(dmph_options *opt);
@@ -209,6 +210,7 @@ typedef struct dmph_info {
     int lanes[2];       /* FP32 sweeps 1 and 2: lanes per particle (dmph_autotune); 1 in FP64 */
     int block[2];       /* FP32 sweeps 1 and 2: threads per workgroup (dmph_autotune); 256 in FP64 */
     double cell_scale;  /* cell width over MaxRadius + 0.1 d (1: the reference's grid) */
+    int wall_lanes;     /* FP32 pass A: lanes per wall row (dmph_autotune); 1 in FP64 */
 } dmph_info;
 int dmph_get_info(dmph_handle *h, dmph_info *info);
 

@@ -346,16 +346,20 @@ def test_fp32_workgroups_and_cell_sizes(name, block, cell_scale):
     s.close(); o.close()
 
 
-def test_static_wall_density_between_full_sweeps():
+@pytest.mark.parametrize("wall_lanes", [1, 2, 4])
+def test_static_wall_density_between_full_sweeps(wall_lanes):
     """Static walls keep their wall neighbours' density sum from the last full
     sweep and walk only their rows' fluid and solid entries on the other steps.
     mix3d's walls stand still; the first two steps sweep everything, steps 3-6
     do not (no rebinning: the skin holds).  After every step the walls' number
     density stays within 1e-5 per particle of the FP64 oracle, as the fluid's,
-    and the walls' PressureP within 1e-4 norm-relative."""
+    and the walls' PressureP within 1e-4 norm-relative -- with 1, 2 or 4 lanes
+    per wall row in pass A (the fluid rows keep their own lane count)."""
     g = G.load("mix3d")
     sc, prm = G.scene(g), G.params(g)
-    s = D.Solver(D.Params.from_dict(prm), sc, precision=D.PREC_FP32, sort_mode=D.SORT_CANONICAL, rigid=False)
+    s = D.Solver(D.Params.from_dict(prm), sc, precision=D.PREC_FP32, sort_mode=D.SORT_CANONICAL, rigid=False,
+                 wall_lanes=wall_lanes)
+    assert s.info().wall_lanes == wall_lanes
     o = P.Oracle(P.params_from_dict(prm), sc, sort_mode=P.SORT_CANONICAL)
     n0p = o.scalars()["n0p"]
     typ = o.field("type")
@@ -646,6 +650,7 @@ def test_autotune_keeps_state_and_tolerance():
     lanes = s.autotune(3)
     assert set(lanes) <= {2, 4, 8, 16} and tuple(s.info().lanes) == lanes
     assert set(s.info().block) <= {64, 128, 256}
+    assert s.info().wall_lanes in (1, 2, 4)
     assert s.time() == t0
     for f in fields:
         np.testing.assert_array_equal(s.field(f), before[f], err_msg=f)
