@@ -97,3 +97,31 @@ def test_create_reports_errors_without_crashing(tmp_path):
     with pytest.raises(D.DmphError) as e:
         D.Solver(D.Params.from_dict(S.params_c1()), sc)
     assert "cell" in str(e.value) or "device" in str(e.value)
+
+
+def test_ctypes_mirrors_match_the_header(tmp_path):
+    """The package's ctypes structures lay out every field where the C header
+    puts it (offsets and sizes from gcc on include/dem_mph.h), so a field
+    appended to a struct on one side only fails here rather than as a
+    misread option on the GPU."""
+    import shutil
+    import subprocess
+    if shutil.which("gcc") is None:
+        pytest.skip("gcc not found")
+    mirrors = {"dmph_params": D.Params, "dmph_grid": D.Grid, "dmph_options": D.Options, "dmph_info": D.Info}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "dem_mph.h"', "int main(void) {"]
+    for cname, py in mirrors.items():
+        lines.append('printf("%s sizeof %%zu\\n", sizeof(%s));' % (cname, cname))
+        for fname, _ in py._fields_:
+            lines.append('printf("%s %s %%zu\\n", offsetof(%s, %s));' % (cname, fname, cname, fname))
+    lines += ["return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {tuple(l.split()[:2]): int(l.split()[2]) for l in out if l.strip()}
+    for cname, py in mirrors.items():
+        assert got[(cname, "sizeof")] == C.sizeof(py), cname
+        for fname, _ in py._fields_:
+            assert got[(cname, fname)] == getattr(py, fname).offset, (cname, fname)
