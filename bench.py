@@ -562,7 +562,8 @@ def main(argv=None) -> int:
     achieved = algo_bytes / avg_s / 1e9
     # the kernels the autotuner picked (the profile holds every candidate's launches)
     lanes = mixed["lanes"]
-    knames = ["k_pass_a_fw<%d, true>" % lanes[0], "k_pass_a_s<%d, true>" % lanes[0], "k_pass_b<%d, true>" % lanes[1]]
+    knames = ["k_pass_a_fw<%d, %d, true>" % (lanes[0], mixed["wall_lanes"]), "k_pass_a_s<%d, true>" % lanes[0],
+              "k_pass_b<%d, true>" % lanes[1]]
     traffic = [kernel_traffic(k) for k in knames]
     step_achieved = pss * STEP_BYTES / 1e9
     out = {
