@@ -738,26 +738,34 @@ class Engine final : public dmph_handle {
         const int cand[4] = {2, 4, 8, 16}, bcand[3] = {64, 128, 256};
         const int orig[2] = {g.lanes[0], g.lanes[1]}, borig[2] = {g.block[0], g.block[1]};
         int picks[2] = {orig[0], orig[1]}, bpicks[2] = {borig[0], borig[1]};
+        // every candidate timed in two rounds, the rounds interleaved, and
+        // scored by the faster of its two: one slow round (a clock or cache
+        // transient) does not decide a pick
         for (int w = 0; w < 2; w++) {
+            for (int c = 0; c < 4; c++)
+                for (int bc = 0; bc < 3; bc++) tune_ms[w][c][bc] = 1e30f;
+            for (int round = 0; round < 2; round++)
+                for (int c = 0; c < 4; c++)
+                    for (int bc = 0; bc < 3; bc++) {
+                        g.lanes[w] = cand[c];
+                        g.block[w] = bcand[bc];
+                        auto launch = [&] {
+                            if (w == 0) launch_sweep1<R>(g, a, stream);
+                            else launch_sweep2<R>(g, a, stream);
+                        };
+                        launch();
+                        HCHK(hipEventRecord(e0, stream));
+                        for (int r = 0; r < reps; r++) launch();
+                        HCHK(hipEventRecord(e1, stream));
+                        HCHK(hipEventSynchronize(e1));
+                        float ms = 0.f;
+                        HCHK(hipEventElapsedTime(&ms, e0, e1));
+                        tune_ms[w][c][bc] = std::min(tune_ms[w][c][bc], ms / reps);
+                    }
             float best = 1e30f;
             for (int c = 0; c < 4; c++)
-                for (int bc = 0; bc < 3; bc++) {
-                    g.lanes[w] = cand[c];
-                    g.block[w] = bcand[bc];
-                    auto launch = [&] {
-                        if (w == 0) launch_sweep1<R>(g, a, stream);
-                        else launch_sweep2<R>(g, a, stream);
-                    };
-                    launch();
-                    HCHK(hipEventRecord(e0, stream));
-                    for (int r = 0; r < reps; r++) launch();
-                    HCHK(hipEventRecord(e1, stream));
-                    HCHK(hipEventSynchronize(e1));
-                    float ms = 0.f;
-                    HCHK(hipEventElapsedTime(&ms, e0, e1));
-                    tune_ms[w][c][bc] = ms / reps;
-                    if (ms < best) { best = ms; picks[w] = cand[c]; bpicks[w] = bcand[bc]; }
-                }
+                for (int bc = 0; bc < 3; bc++)
+                    if (tune_ms[w][c][bc] < best) { best = tune_ms[w][c][bc]; picks[w] = cand[c]; bpicks[w] = bcand[bc]; }
             g.lanes[w] = orig[w];
             g.block[w] = borig[w];
         }
@@ -768,20 +776,25 @@ class Engine final : public dmph_handle {
         int wpick = worig;
         if (g.fused && !g.st_active && g.we > g.wb) {
             const int wcand[3] = {1, 2, 4};
-            float best = 1e30f;
+            float wms[3] = {1e30f, 1e30f, 1e30f};
             g.lanes[0] = picks[0];
             g.block[0] = bpicks[0];
-            for (int c = 0; c < 3; c++) {
-                g.lanes_w = wcand[c];
-                launch_sweep1<R>(g, a, stream);
-                HCHK(hipEventRecord(e0, stream));
-                for (int r = 0; r < reps; r++) launch_sweep1<R>(g, a, stream);
-                HCHK(hipEventRecord(e1, stream));
-                HCHK(hipEventSynchronize(e1));
-                float ms = 0.f;
-                HCHK(hipEventElapsedTime(&ms, e0, e1));
-                if (ms < best) { best = ms; wpick = wcand[c]; }
-            }
+            for (int round = 0; round < 2; round++)
+                for (int c = 0; c < 3; c++) {
+                    g.lanes_w = wcand[c];
+                    launch_sweep1<R>(g, a, stream);
+                    HCHK(hipEventRecord(e0, stream));
+                    for (int r = 0; r < reps; r++) launch_sweep1<R>(g, a, stream);
+                    HCHK(hipEventRecord(e1, stream));
+                    HCHK(hipEventSynchronize(e1));
+                    float ms = 0.f;
+                    HCHK(hipEventElapsedTime(&ms, e0, e1));
+                    wms[c] = std::min(wms[c], ms);
+                }
+            int bw = 0;
+            for (int c = 1; c < 3; c++)
+                if (wms[c] < wms[bw]) bw = c;
+            wpick = wcand[bw];
             g.lanes[0] = orig[0];
             g.block[0] = borig[0];
             g.lanes_w = worig;
