@@ -100,6 +100,7 @@ struct Dev {
     int *key_new, *order, *perm;   // [n]
     int *bkey, *bval;              // [p2] bitonic
     int *block_sums;               // scan carries
+    int *wall_counts;              // [T] wall cells' counts at the last rebinning that counted the walls
     int *t_int;                    // [5n] gather scratch
     R *t_real;                     // [8n]
     double *t_dbl;                 // [3n]

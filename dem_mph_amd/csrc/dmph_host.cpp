@@ -416,6 +416,7 @@ class Engine final : public dmph_handle {
         TRY(alloc(a.key_new, n)); TRY(alloc(a.order, n)); TRY(alloc(a.perm, n));
         if (g.sort_reference) { TRY(alloc(a.bkey, (size_t)g.p2)); TRY(alloc(a.bval, (size_t)g.p2)); }
         TRY(alloc(a.block_sums, nk / 1024 + 2));
+        TRY(alloc(a.wall_counts, (size_t)g.T + 1));
         TRY(alloc(a.t_int, 5 * n)); TRY(alloc(a.t_real, 8 * n));
         TRY(alloc(a.t_dbl, g.rigid ? 3 * n : 1)); TRY(alloc(a.r0, g.rigid ? 3 * n : 1));
         TRY(alloc(a.sc, 1));
