@@ -76,6 +76,8 @@ struct Scalars {
     int walls_moving;               // this step moved the walls (0.2 <= Time <= 0.4, written by k_predict)
     int walls_dirty;                // walls moved (or the host set positions) since the last rebinning
     int build_walls_dirty;          // ... as of this rebinning: the list build re-walks every wall row
+    int walls_rest;                 // every wall's velocity is zero: the last step moved no wall (set by the
+                                    // rebin decision) and the host has written no state since (cleared)
 };
 
 template <class R>

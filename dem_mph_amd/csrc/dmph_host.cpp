@@ -1022,8 +1022,9 @@ class Engine final : public dmph_handle {
             HCHK(hipMemcpyAsync(&a.sc->force, &one, sizeof one, hipMemcpyHostToDevice, stream));
         }
         {   // the next step's sweep 1 covers every particle (walls may have been moved or set in motion)
-            const int two = 2;
+            const int two = 2, zero = 0;
             HCHK(hipMemcpyAsync(&a.sc->full_sweep, &two, sizeof two, hipMemcpyHostToDevice, stream));
+            HCHK(hipMemcpyAsync(&a.sc->walls_rest, &zero, sizeof zero, hipMemcpyHostToDevice, stream));
         }
         HCHK(hipStreamSynchronize(stream));   // host vectors go out of scope
         return DMPH_OK;
