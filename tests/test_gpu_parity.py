@@ -262,6 +262,37 @@ def test_fp32_multi_step_stays_close(name):
     assert verr <= 1e-3
 
 
+@pytest.mark.parametrize("start", [0.2, 0.4])
+def test_fp32_walls_through_the_motion_window_edge(start):
+    """wall2d's walls move while 0.2 <= Time <= 0.4 s (source/DEM-MPH.cpp:
+    2846-2900).  Started 3 steps before an edge of that window and run 8 steps
+    with the FP64 oracle beside it: the walls start moving, or stop and are
+    zeroed, then stand still (the FP32 step skips walls at rest in the
+    prediction, and leaves static walls out of the sort).  Every wall position
+    matches the oracle's to FP32 rounding of the cell-relative offsets, and
+    the state stays as close as over the golden run."""
+    sc, prm, L = _case("wall2d")
+    sc.time = start - 3 * prm["Dt"]
+    s = D.Solver(D.Params.from_dict(prm), sc, precision=D.PREC_FP32, sort_mode=D.SORT_CANONICAL)
+    o = P.Oracle(P.params_from_dict(prm), sc, sort_mode=P.SORT_CANONICAL)
+    s.step(8); o.step(8)
+    typ = o.field("type")
+    wall = typ >= 4
+    assert wall.any()
+    dx = s.field("pos") - o.field("pos")
+    dx = dx - L * np.round(dx / L)
+    v, vr = s.field("vel"), o.field("vel")
+    verr = np.linalg.norm(v - vr) / np.linalg.norm(vr)
+    cw = np.array(s.info().cell_width[:])
+    _report("wall2d-edge-%g" % start, {"wall_pos_abs": np.abs(dx[wall]).max(), "pos_abs": np.abs(dx).max(),
+                                       "vel_rel": verr})
+    assert np.abs(dx[wall]).max() <= 2.0 ** -21 * cw.max()
+    assert np.abs(dx).max() <= 1e-7
+    assert verr <= 1e-3
+    np.testing.assert_array_equal(v[wall] == 0.0, vr[wall] == 0.0)
+    s.close(); o.close()
+
+
 def _rigid_scene():
     g = G.load("mix3d")
     sc = G.scene(g)
