@@ -405,7 +405,7 @@ def test_folded_half_edge_sums_match_fp64(S, reach):
     subtracts Wc3 P_i from A_i and multiplies only the distance per edge.  The
     sums equal the FP64 sums of the unfolded formula to the unfolded kernel's
     tolerance, also with the origin moved so that |P| reaches 16 radii (the
-    driver's limit); rows cut to i % 71 entries (empty, tails of every length)."""
+    limit the surrogate's host code keeps); rows cut to i % 71 entries (empty, tails of every length)."""
     sur, sc, pos, vh, cls, tgt, mask = _small_problem(S, seed=3)
     n = sc.n
     ci, deg, cap, dist = sur.graph_rows(pos, row_cap=16, dist=True)
