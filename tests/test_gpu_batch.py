@@ -141,3 +141,35 @@ def test_batch_fp32_rigid_within_stated_tolerance_of_oracle():
         boff += len(bx)
         o.close()
     assert boff == batch.info().nbodies
+
+
+def test_batch_fp32_equals_each_scene_alone_across_the_wall_motion_window():
+    """FP32, rebinning every step, started 3 steps before the walls' motion
+    window opens (Time 0.2 s, source/DEM-MPH.cpp:2846-2900): the walls rest
+    (their slots skipped by the prediction and left out of the sort), then
+    are moved by the window's transform, then rebinned with everything else.
+    Every scene of the batch ends where its own handle does, field for field:
+    the batch's flags and saved wall counts serve all scenes at once."""
+    scenes = _batch(3, 5, rigid=True)
+    prm = S.params_c1()
+    for sc in scenes:
+        sc.time = 0.2 - 3 * prm["Dt"]
+    p = D.Params.from_dict(prm)
+    kw = dict(precision=D.PREC_FP32, sort_mode=D.SORT_CANONICAL, rebuild=D.REBUILD_EVERY_STEP, rigid=True)
+    batch = D.Solver(p, scenes, **kw)
+    alone = [D.Solver(p, sc, **kw) for sc in scenes]
+    batch.step(8)
+    for s in alone:
+        s.step(8)
+    off = batch.scene_offsets
+    for f in ("pos", "vel", "force", "vol_strain_p", "pressure_p", "kn", "neighbor_count", "id"):
+        fb = batch.field(f)
+        for k, s in enumerate(alone):
+            ref = s.field(f)
+            if f == "id":
+                ref = ref + off[k]
+            np.testing.assert_array_equal(fb[off[k]:off[k + 1]], ref, err_msg="%s scene %d" % (f, k))
+    assert batch.time() == alone[0].time() and batch.time() > 0.2
+    batch.close()
+    for s in alone:
+        s.close()
